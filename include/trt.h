@@ -1,0 +1,207 @@
+/* trt — MI355X (gfx950) wavefront path tracer, C ABI.
+ *
+ * One opaque context owns a HIP stream and every device buffer. Scene data is
+ * copied in from caller host arrays; results are copied out to caller host arrays
+ * (the *_device variants take device pointers and run on the context's stream).
+ * Calls are synchronous, never abort, and report failures through trt_status plus
+ * trt_last_error(). A context is not thread-safe (callers serialise access).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference repository root).
+ */
+#ifndef TRT_H
+#define TRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum trt_status {
+  TRT_OK = 0,
+  TRT_INVALID_ARG = 1,
+  TRT_OUT_OF_MEMORY = 2,
+  TRT_DEVICE = 3,
+  TRT_NOT_BUILT = 4,
+  TRT_UNSUPPORTED = 5
+} trt_status;
+
+typedef struct trt_context trt_context;
+
+/* Material (replaces CudaMatarial's 13 properties, Tracer/Common/CommonStructs.h:76-91,
+ * 288-307; Disney parameters as in Tracer/Resources/Material.cpp:98-115). `emission`
+ * is radiance; model 0 = Disney, 1 = Lambert. */
+typedef struct trt_material {
+  float base_color[3];
+  float roughness;
+  float emission[3];
+  float metallic;
+  float specular;
+  float specular_tint;
+  float sheen;
+  float sheen_tint;
+  float clearcoat;
+  float clearcoat_gloss;
+  float subsurface;
+  float anisotropic;
+  uint32_t model;
+} trt_material;
+
+/* Instance of a mesh (replaces Instance::InstanceData, Tracer/Resources/Instance.cpp:39-49):
+ * row-major 3x4 object-to-world transform. */
+typedef struct trt_instance {
+  float transform[12];
+  uint32_t mesh;
+  uint32_t material_offset;
+} trt_instance;
+
+/* Camera (replaces DeviceRenderer::SetCamera, Tracer/Renderer/DeviceRenderer.cpp:25-40,
+ * and the CameraNode fields it reads). fov is horizontal, in radians. */
+typedef struct trt_camera {
+  float position[3];
+  float target[3];
+  float up[3];
+  float fov;
+  float aperture;
+  float focal_dist;
+  float distortion;
+  int32_t bokeh_sides;
+  float bokeh_rotation;
+} trt_camera;
+
+/* BVH build options (replaces the OptiX build inputs, Tracer/Resources/Model.cpp:240-306,
+ * Tracer/Optix/OptixRenderer.cpp:90-123). */
+typedef struct trt_build_opts {
+  int32_t bins;      /* binned-SAH bins per axis (32) */
+  int32_t max_leaf;  /* primitives per leaf, 1..7 (4) */
+  int32_t threads;   /* host build threads (0 = hardware concurrency) */
+} trt_build_opts;
+
+/* Render options (replaces KernelSettings, Tracer/Common/CommonStructs.h:145-153, plus the
+ * film size). One trt_render call traces `spp` samples per pixel (the reference's
+ * multiSample) and adds them to the accumulator. */
+typedef struct trt_render_opts {
+  uint32_t width;
+  uint32_t height;
+  uint32_t spp;
+  uint32_t max_depth;
+  uint32_t rr_depth;   /* Russian roulette applies at vertices of depth >= rr_depth */
+  uint32_t seed;
+  float ray_epsilon;
+  float clamp;         /* firefly clamp on a contribution's max component; 0 = off */
+} trt_render_opts;
+
+/* Frame statistics (replaces RenderStatistics::DeviceStatistics,
+ * Tracer/Renderer/RenderStatistics.h:11-26). Times are HIP-event milliseconds. */
+typedef struct trt_stats {
+  uint64_t paths;
+  uint64_t extend_rays;   /* all closest-hit rays, primary included */
+  uint64_t shadow_rays;
+  uint64_t primary_rays;
+  float raygen_ms;
+  float extend_ms;        /* sum over bounces of Extend kernel time */
+  float shade_ms;
+  float shadow_ms;
+  float finalize_ms;
+  float frame_ms;         /* first to last event of the frame */
+  uint32_t extend_launches;
+  uint32_t shadow_launches;
+  uint32_t sample_count;  /* accumulated samples per pixel after this frame */
+  uint32_t overflow;      /* nonzero if a traversal stack overflowed (must stay 0) */
+} trt_stats;
+
+typedef struct trt_bvh_info {
+  uint64_t nodes;
+  uint64_t leaves;
+  uint32_t depth;
+  uint32_t stack_entries;
+  double sah_cost;
+  double build_ms;
+  uint64_t device_bytes;
+  uint32_t two_level;
+  uint32_t tlas_depth;
+} trt_bvh_info;
+
+typedef struct trt_pick_result {
+  float origin[3];
+  float direction[3];
+  float t;
+  uint32_t instance;
+  uint32_t primitive;
+} trt_pick_result;
+
+/* Context life cycle (replaces Renderer::Renderer / ~Renderer, Tracer/Renderer/Renderer.cpp:41-66,
+ * and CudaDevice(0), Tracer/CUDA/CudaDevice.cpp). */
+trt_status trt_create(int device, trt_context** out);
+void trt_destroy(trt_context* ctx);
+const char* trt_last_error(const trt_context* ctx);
+const char* trt_version(void);
+
+/* Scene (replaces Renderer::UpdateScene's geometry/material upload, Tracer/Renderer/Renderer.cpp:79-143,
+ * 442-504, with Model::Build packing, Tracer/Resources/Model.cpp:93-167). Triangle soup:
+ * positions 9 floats per triangle, normals 9 floats per triangle or NULL, one material
+ * id per triangle. */
+trt_status trt_clear_scene(trt_context* ctx);
+trt_status trt_set_materials(trt_context* ctx, const trt_material* mats, uint32_t n);
+trt_status trt_add_mesh(trt_context* ctx, const float* positions, const float* normals,
+                        const uint32_t* material_ids, uint32_t n_tris, uint32_t* mesh_id);
+/* Two-level scenes: when instances are set, every mesh is traced through them. */
+trt_status trt_set_instances(trt_context* ctx, const trt_instance* inst, uint32_t n);
+/* Builds the BVH(s), the light table, and uploads everything (replaces Model::Upload +
+ * OptixRenderer::BuildAccel + Scene::GatherLights, Tracer/Renderer/Scene.cpp:275-302). */
+trt_status trt_build(trt_context* ctx, const trt_build_opts* opts);
+trt_status trt_bvh_info_get(trt_context* ctx, trt_bvh_info* info);
+
+/* Camera and frame (replaces Renderer::SetCamera / RenderFrame / Reset,
+ * Tracer/Renderer/Renderer.h:45-61; DeviceRenderer::RenderFrame, Tracer/Renderer/DeviceRenderer.cpp:97-133). */
+trt_status trt_set_camera(trt_context* ctx, const trt_camera* cam);
+trt_status trt_render(trt_context* ctx, const trt_render_opts* opts, trt_stats* stats);
+trt_status trt_reset(trt_context* ctx);
+uint32_t trt_sample_count(const trt_context* ctx);
+
+/* Outputs (replace the accumulator / colour / albedo / normal buffers,
+ * Tracer/Renderer/DeviceRenderer.h:92-95, FinalizeFrame, Tracer/CUDA/GPU/CudaUtilityKernels.h:27-44).
+ * Each writes width*height*4 floats; accum = (rgb sum, sample count), color = mean,
+ * albedo = first-hit albedo, normal = (first-hit shading normal xyz, view depth). */
+trt_status trt_read_accum(trt_context* ctx, float* rgba);
+trt_status trt_read_color(trt_context* ctx, float* rgba);
+trt_status trt_read_albedo(trt_context* ctx, float* rgba);
+trt_status trt_read_normal(trt_context* ctx, float* rgba);
+/* Per-path radiance of the last frame, 3 floats per path (path = sample*W*H + pixel). */
+trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths);
+
+/* Ray queries on the built scene (replace optixTrace closest / terminate-on-first-hit,
+ * Tracer/Optix/GPU/optix.cu:264-265, 333-335). rays: 8 floats per ray
+ * (ox, oy, oz, tmin, dx, dy, dz, tmax). hits: 4 floats per ray (t, b1, b2, prim as uint
+ * bits; prim = 0xffffffff on a miss); inst: instance per ray (0xffffffff single-level). */
+trt_status trt_trace_closest(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst);
+trt_status trt_trace_any(trt_context* ctx, const float* rays, uint32_t n, uint8_t* occluded);
+
+/* Ray pick (replaces DeviceRenderer::PickRay, Tracer/Renderer/DeviceRenderer.cpp:71-93, with the
+ * RayPick ray-gen mode, Tracer/Optix/GPU/optix.cu:345-377): camera ray through the
+ * centre of pixel (x, y) (y = 0 bottom row), no jitter. */
+trt_status trt_pick(trt_context* ctx, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
+                    trt_pick_result* out);
+
+/* Measurement hook: re-traces the last frame's bounce-0/1 extension and shadow queues
+ * with counting kernels and reports average node visits and triangle tests per ray
+ * (used for the algorithmic-byte roofline). */
+typedef struct trt_trace_counts {
+  double closest_nodes_per_ray;
+  double closest_tris_per_ray;
+  double any_nodes_per_ray;
+  double any_tris_per_ray;
+  uint64_t closest_rays;
+  uint64_t any_rays;
+  uint32_t node_bytes;
+  uint32_t tri_bytes;
+} trt_trace_counts;
+trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* opts, trt_trace_counts* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRT_H */

@@ -1,0 +1,800 @@
+// C-ABI implementation: context, scene upload, BVH build, frame orchestration.
+//
+// Replaces the reference's Renderer / DeviceRenderer / OptixRenderer orchestration
+// (Tracer/Renderer/Renderer.cpp:79-190, 442-504; Tracer/Renderer/DeviceRenderer.cpp:97-274;
+// Tracer/Optix/OptixRenderer.cpp:90-131). Differences by design: all device state is
+// owned by the context (no __constant__ pointer globals, CudaGlobals.h:17-38), one HIP
+// stream per context, no host synchronisation between bounces (queue sizes stay on the
+// device; the reference downloads counters every bounce, DeviceRenderer.cpp:256), and
+// errors are returned as trt_status instead of exit() (Tracer/Utility/Errors.cpp:15-37).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/trt.h"
+#include "bvh_build.h"
+#include "device.h"
+#include "trt/lights_build.h"
+
+using namespace trt;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    bytes = n;
+    return hipSuccess;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct Mesh {
+  uint32_t tri_offset, n_tris;
+};
+
+}  // namespace
+
+struct trt_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  std::string err;
+
+  // host scene
+  std::vector<float> pos, nrm;
+  std::vector<uint8_t> has_nrm;  // per mesh
+  std::vector<uint32_t> mat_ids;
+  std::vector<Mesh> meshes;
+  std::vector<Material> materials;
+  std::vector<trt_instance> instances_in;
+  bool dirty = true;
+  bool built = false;
+
+  // device scene
+  DevBuf d_pos, d_nrm, d_mat_ids, d_materials, d_inst, d_inst_tlas, d_lights, d_alias_p, d_alias_i;
+  DevBuf d_nodes, d_tris, d_tlas;
+  uint32_t n_lights = 0;
+  float total_power = 0.f;
+  bool use_normals = false;
+  bool two_level = false;
+  int stack_cap = 1;
+  trt_bvh_info info{};
+
+  // camera / film
+  bool camera_set = false;
+  CameraParams cam{};
+  uint32_t width = 0, height = 0;
+  uint32_t sample_count = 0;
+  uint64_t last_paths = 0;
+
+  // frame buffers
+  DevBuf q_o[2], q_d[2], q_t[2], q_m[2];
+  DevBuf h_t, h_i;
+  DevBuf s_o, s_d, s_c;
+  DevBuf radiance;
+  DevBuf accum, color, albedo, normal;
+  DevBuf counters;
+  DevBuf scratch_rays, scratch_hits, scratch_inst, scratch_occ;
+  uint64_t path_capacity = 0;
+
+  hipEvent_t ev[4 + 4 * kMaxDepth] = {};
+  int grid_trace = 0, grid_shade = 0;
+};
+
+namespace {
+
+trt_status fail(trt_context* c, trt_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+trt_status hip_fail(trt_context* c, hipError_t e, const char* what) {
+  const trt_status s = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? TRT_OUT_OF_MEMORY : TRT_DEVICE;
+  return fail(c, s, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPC(expr, what)                           \
+  do {                                             \
+    hipError_t e_ = (expr);                        \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+  } while (0)
+
+hipError_t upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
+  hipError_t e = b.ensure(std::max<size_t>(bytes, 16));
+  if (e != hipSuccess) return e;
+  if (bytes) return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
+  return hipSuccess;
+}
+
+Material to_material(const trt_material& m) {
+  Material r{};
+  for (int k = 0; k < 3; ++k) {
+    r.base[k] = m.base_color[k];
+    r.emission[k] = m.emission[k];
+  }
+  r.roughness = m.roughness;
+  r.metallic = m.metallic;
+  r.specular = m.specular;
+  r.specular_tint = m.specular_tint;
+  r.sheen = m.sheen;
+  r.sheen_tint = m.sheen_tint;
+  r.clearcoat = m.clearcoat;
+  r.clearcoat_gloss = m.clearcoat_gloss;
+  r.subsurface = m.subsurface;
+  r.anisotropic = m.anisotropic;
+  r.model = m.model;
+  return r;
+}
+
+bool finite_all(const float* p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!(p[i] == p[i]) || p[i] == __builtin_huge_valf() || p[i] == -__builtin_huge_valf()) return false;
+  return true;
+}
+
+void tri_boxes(const float* pos, uint32_t first, uint32_t n, std::vector<float>& box) {
+  box.resize(size_t(n) * 6);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* p = pos + size_t(first + i) * 9;
+    for (int k = 0; k < 3; ++k) {
+      box[size_t(i) * 6 + k] = std::min(p[k], std::min(p[3 + k], p[6 + k]));
+      box[size_t(i) * 6 + 3 + k] = std::max(p[k], std::max(p[3 + k], p[6 + k]));
+    }
+  }
+}
+
+// Appends a mesh BVH in device layout: node indices offset by node_base, leaf
+// triangle ranges offset by tri_base; traversal triangles get mesh-local prim ids.
+void append_blas(const float* pos, uint32_t tri_offset, const BuildResult& b, std::vector<DevNode2>& nodes,
+                 std::vector<float>& tris) {
+  std::vector<DevNode2> dn = to_device_layout(b);
+  const int32_t node_base = int32_t(nodes.size());
+  const uint32_t tri_base = uint32_t(tris.size() / 12);
+  for (DevNode2& n : dn) {
+    for (int s = 0; s < 2; ++s) {
+      if (n.c[s] >= 0) {
+        n.c[s] += node_base;
+      } else {
+        const uint32_t code = ~uint32_t(n.c[s]);
+        const uint32_t first = (code >> 3) + tri_base, cnt = code & 7u;
+        n.c[s] = leaf_code(first, cnt);
+      }
+    }
+    nodes.push_back(n);
+  }
+  for (uint32_t k = 0; k < b.prim_order.size(); ++k) {
+    const uint32_t local = b.prim_order[k];
+    const float* p = pos + size_t(tri_offset + local) * 9;
+    float rec[12] = {p[0], p[1], p[2], 0.f, p[3], p[4], p[5], 0.f, p[6], p[7], p[8], 0.f};
+    std::memcpy(&rec[3], &local, 4);
+    tris.insert(tris.end(), rec, rec + 12);
+  }
+}
+
+trt_status ensure_frame(trt_context* ctx, uint32_t W, uint32_t H, uint32_t spp) {
+  const uint64_t n = uint64_t(W) * H * spp;
+  if (n == 0 || n >= (1ull << 30)) return fail(ctx, TRT_INVALID_ARG, "path count must be in [1, 2^30)");
+  const uint64_t npix = uint64_t(W) * H;
+  if (W != ctx->width || H != ctx->height) {
+    HIPC(ctx->accum.ensure(npix * 16), "alloc accum");
+    HIPC(ctx->color.ensure(npix * 16), "alloc color");
+    HIPC(ctx->albedo.ensure(npix * 16), "alloc albedo");
+    HIPC(ctx->normal.ensure(npix * 16), "alloc normal");
+    HIPC(hipMemsetAsync(ctx->albedo.p, 0, npix * 16, ctx->stream), "clear albedo");
+    HIPC(hipMemsetAsync(ctx->normal.p, 0, npix * 16, ctx->stream), "clear normal");
+    ctx->width = W;
+    ctx->height = H;
+    ctx->sample_count = 0;
+  }
+  if (n > ctx->path_capacity) {
+    for (int i = 0; i < 2; ++i) {
+      HIPC(ctx->q_o[i].ensure(n * 16), "alloc queue");
+      HIPC(ctx->q_d[i].ensure(n * 16), "alloc queue");
+      HIPC(ctx->q_t[i].ensure(n * 16), "alloc queue");
+      HIPC(ctx->q_m[i].ensure(n * 4), "alloc queue");
+    }
+    HIPC(ctx->h_t.ensure(n * 16), "alloc hits");
+    HIPC(ctx->h_i.ensure(n * 4), "alloc hits");
+    HIPC(ctx->s_o.ensure(n * 16), "alloc shadow queue");
+    HIPC(ctx->s_d.ensure(n * 16), "alloc shadow queue");
+    HIPC(ctx->s_c.ensure(n * 16), "alloc shadow queue");
+    HIPC(ctx->radiance.ensure(n * 16), "alloc radiance");
+    ctx->path_capacity = n;
+  }
+  return TRT_OK;
+}
+
+DevAccel accel_of(const trt_context* ctx) {
+  DevAccel a;
+  a.nodes = ctx->d_nodes.as<DevNode2>();
+  a.tris = ctx->d_tris.as<float4>();
+  a.tlas = ctx->two_level ? ctx->d_tlas.as<DevNode2>() : nullptr;
+  a.inst = ctx->two_level ? ctx->d_inst_tlas.as<Instance>() : nullptr;
+  a.stack_cap = ctx->stack_cap;
+  a.two_level = ctx->two_level ? 1 : 0;
+  return a;
+}
+
+SceneView view_of(const trt_context* ctx) {
+  SceneView s{};
+  s.tri_pos = ctx->d_pos.as<float>();
+  s.tri_nrm = ctx->use_normals ? ctx->d_nrm.as<float>() : nullptr;
+  s.tri_mat = ctx->d_mat_ids.as<uint32_t>();
+  s.materials = ctx->d_materials.as<Material>();
+  s.instances = ctx->two_level ? ctx->d_inst.as<Instance>() : nullptr;
+  s.lights = ctx->d_lights.as<LightTri>();
+  s.alias_prob = ctx->d_alias_p.as<float>();
+  s.alias_idx = ctx->d_alias_i.as<uint32_t>();
+  s.n_lights = ctx->n_lights;
+  s.total_power = ctx->total_power;
+  s.n_tris = uint32_t(ctx->mat_ids.size());
+  s.n_materials = uint32_t(ctx->materials.size());
+  return s;
+}
+
+RenderParams params_of_opts(const trt_context* ctx, const trt_render_opts& o) {
+  RenderParams rp{};
+  rp.cam = ctx->cam;
+  rp.width = o.width;
+  rp.height = o.height;
+  rp.spp = o.spp;
+  rp.max_depth = o.max_depth;
+  rp.rr_depth = o.rr_depth;
+  rp.seed = o.seed;
+  rp.sample_offset = ctx->sample_count;
+  rp.eps = o.ray_epsilon;
+  rp.clamp = o.clamp;
+  return rp;
+}
+
+// Launches one frame on ctx->stream. stats = counting traversal kernels (no timing use).
+trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, bool accumulate, trt_stats* st) {
+  trt_status s = ensure_frame(ctx, o.width, o.height, o.spp);
+  if (s != TRT_OK) return s;
+  const RenderParams rp = params_of_opts(ctx, o);
+  const uint32_t n = o.width * o.height * o.spp;
+  const DevAccel acc = accel_of(ctx);
+  const SceneView sv = view_of(ctx);
+  uint32_t* ctr = ctx->counters.as<uint32_t>();
+  FrameBuffers fb;
+  fb.radiance = ctx->radiance.as<float4>();
+  fb.accum = ctx->accum.as<float4>();
+  fb.color = ctx->color.as<float4>();
+  fb.albedo = ctx->albedo.as<float4>();
+  fb.normal = ctx->normal.as<float4>();
+  RayQueue q[2];
+  for (int i = 0; i < 2; ++i) {
+    q[i].o_tmin = ctx->q_o[i].as<float4>();
+    q[i].d_tmax = ctx->q_d[i].as<float4>();
+    q[i].t_pdf = ctx->q_t[i].as<float4>();
+    q[i].meta = ctx->q_m[i].as<uint32_t>();
+  }
+  HitBuf hb{ctx->h_t.as<float4>(), ctx->h_i.as<uint32_t>()};
+  ShadowQueue sq{ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>()};
+  hipStream_t stm = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  const uint32_t D = std::min<uint32_t>(o.max_depth, kMaxDepth);
+
+  launch_zero_counters(ctr, stm);
+  HIPC(hipEventRecord(ev[0], stm), "event");
+  launch_raygen(rp, n, q[0], fb.radiance, ctr, stm);
+  HIPC(hipEventRecord(ev[1], stm), "event");
+  for (uint32_t d = 0; d < D; ++d) {
+    const int cur = int(d & 1u), nxt = cur ^ 1;
+    launch_extend(acc, q[cur], hb, &ctr[kCtrExt + d], &ctr[kCtrWorkExt + d], ctr, ctx->grid_trace, stats, stm);
+    HIPC(hipEventRecord(ev[4 + 4 * d + 0], stm), "event");
+    launch_shade(sv, ctx->d_materials.as<Material>(), rp, q[cur], hb, q[nxt], sq, fb, &ctr[kCtrExt + d],
+                 &ctr[kCtrWorkShade + d], &ctr[kCtrExt + d + 1], &ctr[kCtrShadow + d], d, ctx->grid_shade, stm);
+    HIPC(hipEventRecord(ev[4 + 4 * d + 1], stm), "event");
+    launch_shadow(acc, sq, fb.radiance, &ctr[kCtrShadow + d], &ctr[kCtrWorkShadow + d], ctr, ctx->grid_trace, stats,
+                  stm);
+    HIPC(hipEventRecord(ev[4 + 4 * d + 2], stm), "event");
+  }
+  if (accumulate) launch_finalize(rp, fb.radiance, fb, ctx->sample_count == 0 ? 1u : 0u, stm);
+  HIPC(hipEventRecord(ev[2], stm), "event");
+  HIPC(hipGetLastError(), "kernel launch");
+  HIPC(hipStreamSynchronize(stm), "frame");
+
+  uint32_t hc[kNumCounters];
+  HIPC(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost), "read counters");
+  if (accumulate) ctx->sample_count += o.spp;
+  ctx->last_paths = n;
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->paths = n;
+    st->primary_rays = hc[kCtrExt + 0];
+    for (uint32_t d = 0; d < D; ++d) {
+      st->extend_rays += hc[kCtrExt + d];
+      st->shadow_rays += hc[kCtrShadow + d];
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    st->raygen_ms = ms;
+    hipEvent_t prev = ev[1];
+    for (uint32_t d = 0; d < D; ++d) {
+      (void)hipEventElapsedTime(&ms, prev, ev[4 + 4 * d + 0]);
+      st->extend_ms += ms;
+      (void)hipEventElapsedTime(&ms, ev[4 + 4 * d + 0], ev[4 + 4 * d + 1]);
+      st->shade_ms += ms;
+      (void)hipEventElapsedTime(&ms, ev[4 + 4 * d + 1], ev[4 + 4 * d + 2]);
+      st->shadow_ms += ms;
+      prev = ev[4 + 4 * d + 2];
+    }
+    (void)hipEventElapsedTime(&ms, prev, ev[2]);
+    st->finalize_ms = ms;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[2]);
+    st->frame_ms = ms;
+    st->extend_launches = D;
+    st->shadow_launches = D;
+    st->sample_count = ctx->sample_count;
+    st->overflow = hc[kCtrOverflow];
+  }
+  if (hc[kCtrOverflow]) return fail(ctx, TRT_DEVICE, "traversal stack overflow");
+  (void)stats;
+  return TRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* trt_version(void) { return "trt 0.1 (gfx950 wavefront path tracer)"; }
+
+static trt_status init_context(trt_context* ctx) {
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
+  hipDeviceProp_t prop;
+  HIPC(hipGetDeviceProperties(&prop, ctx->device), "device properties");
+  ctx->num_cus = prop.multiProcessorCount;
+  for (auto& e : ctx->ev) HIPC(hipEventCreate(&e), "event");
+  HIPC(ctx->counters.ensure(kNumCounters * 4), "alloc counters");
+  HIPC(hipMemset(ctx->counters.p, 0, kNumCounters * 4), "clear counters");
+  return TRT_OK;
+}
+
+trt_status trt_create(int device, trt_context** out) {
+  if (!out) return TRT_INVALID_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TRT_DEVICE;
+  if (device < 0 || device >= ndev) return TRT_INVALID_ARG;
+  trt_context* ctx = new trt_context();
+  ctx->device = device;
+  const trt_status s = init_context(ctx);
+  if (s != TRT_OK) {
+    trt_destroy(ctx);
+    return s;
+  }
+  *out = ctx;
+  return TRT_OK;
+}
+
+void trt_destroy(trt_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* trt_last_error(const trt_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+trt_status trt_clear_scene(trt_context* ctx) {
+  if (!ctx) return TRT_INVALID_ARG;
+  ctx->pos.clear();
+  ctx->nrm.clear();
+  ctx->has_nrm.clear();
+  ctx->mat_ids.clear();
+  ctx->meshes.clear();
+  ctx->instances_in.clear();
+  ctx->built = false;
+  ctx->dirty = true;
+  return TRT_OK;
+}
+
+trt_status trt_set_materials(trt_context* ctx, const trt_material* mats, uint32_t n) {
+  if (!ctx) return TRT_INVALID_ARG;
+  if (n && !mats) return fail(ctx, TRT_INVALID_ARG, "materials is null");
+  std::vector<Material> m(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!finite_all(mats[i].base_color, 3) || !finite_all(mats[i].emission, 3) ||
+        !finite_all(&mats[i].roughness, 1) || !finite_all(&mats[i].metallic, 1))
+      return fail(ctx, TRT_INVALID_ARG, "material " + std::to_string(i) + " has a non-finite value");
+    if (mats[i].model > 1) return fail(ctx, TRT_INVALID_ARG, "unknown material model");
+    m[i] = to_material(mats[i]);
+  }
+  ctx->materials = std::move(m);
+  ctx->built = false;
+  return TRT_OK;
+}
+
+trt_status trt_add_mesh(trt_context* ctx, const float* positions, const float* normals,
+                        const uint32_t* material_ids, uint32_t n_tris, uint32_t* mesh_id) {
+  if (!ctx) return TRT_INVALID_ARG;
+  if (n_tris && (!positions || !material_ids)) return fail(ctx, TRT_INVALID_ARG, "positions/material_ids is null");
+  if (!finite_all(positions, size_t(n_tris) * 9)) return fail(ctx, TRT_INVALID_ARG, "non-finite vertex position");
+  if (normals && !finite_all(normals, size_t(n_tris) * 9)) return fail(ctx, TRT_INVALID_ARG, "non-finite normal");
+  const uint64_t total = uint64_t(ctx->mat_ids.size()) + n_tris;
+  if (total >= (1ull << 28)) return fail(ctx, TRT_INVALID_ARG, "more than 2^28 triangles");
+  Mesh m{uint32_t(ctx->mat_ids.size()), n_tris};
+  ctx->pos.insert(ctx->pos.end(), positions, positions + size_t(n_tris) * 9);
+  if (normals) {
+    ctx->nrm.resize(ctx->pos.size(), 0.f);
+    std::memcpy(&ctx->nrm[size_t(m.tri_offset) * 9], normals, size_t(n_tris) * 9 * 4);
+  } else {
+    ctx->nrm.resize(ctx->pos.size(), 0.f);
+  }
+  ctx->has_nrm.push_back(normals ? 1 : 0);
+  ctx->mat_ids.insert(ctx->mat_ids.end(), material_ids, material_ids + n_tris);
+  ctx->meshes.push_back(m);
+  if (mesh_id) *mesh_id = uint32_t(ctx->meshes.size() - 1);
+  ctx->built = false;
+  return TRT_OK;
+}
+
+trt_status trt_set_instances(trt_context* ctx, const trt_instance* inst, uint32_t n) {
+  if (!ctx) return TRT_INVALID_ARG;
+  if (n && !inst) return fail(ctx, TRT_INVALID_ARG, "instances is null");
+  for (uint32_t i = 0; i < n; ++i)
+    if (!finite_all(inst[i].transform, 12)) return fail(ctx, TRT_INVALID_ARG, "non-finite instance transform");
+  ctx->instances_in.assign(inst, inst + n);
+  ctx->built = false;
+  return TRT_OK;
+}
+
+trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
+  if (!ctx) return TRT_INVALID_ARG;
+  trt_build_opts o = opts ? *opts : trt_build_opts{32, 4, 0};
+  if (o.bins < 2 || o.bins > 256 || o.max_leaf < 1 || o.max_leaf > 7)
+    return fail(ctx, TRT_INVALID_ARG, "bins must be in [2,256] and max_leaf in [1,7]");
+  int threads = o.threads > 0 ? o.threads : int(std::thread::hardware_concurrency());
+  if (threads <= 0) threads = 1;
+  const uint32_t n_tris = uint32_t(ctx->mat_ids.size());
+  const uint32_t n_mats = uint32_t(ctx->materials.size());
+  if (n_mats == 0) return fail(ctx, TRT_INVALID_ARG, "no materials");
+  for (uint32_t g = 0; g < n_tris; ++g)
+    if (ctx->mat_ids[g] >= n_mats && ctx->instances_in.empty())
+      return fail(ctx, TRT_INVALID_ARG, "material id out of range at triangle " + std::to_string(g));
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  const auto t0 = std::chrono::steady_clock::now();
+
+  // normals: use them when any mesh provides them; fill the others with face normals
+  ctx->use_normals = false;
+  for (uint8_t h : ctx->has_nrm) ctx->use_normals = ctx->use_normals || h;
+  if (ctx->use_normals) {
+    for (size_t mi = 0; mi < ctx->meshes.size(); ++mi) {
+      if (ctx->has_nrm[mi]) continue;
+      const Mesh& m = ctx->meshes[mi];
+      for (uint32_t k = 0; k < m.n_tris; ++k) {
+        const float* p = &ctx->pos[size_t(m.tri_offset + k) * 9];
+        const V3 n = normalize(cross(v3(p[3] - p[0], p[4] - p[1], p[5] - p[2]), v3(p[6] - p[0], p[7] - p[1], p[8] - p[2])));
+        for (int v = 0; v < 3; ++v) {
+          ctx->nrm[size_t(m.tri_offset + k) * 9 + v * 3 + 0] = n.x;
+          ctx->nrm[size_t(m.tri_offset + k) * 9 + v * 3 + 1] = n.y;
+          ctx->nrm[size_t(m.tri_offset + k) * 9 + v * 3 + 2] = n.z;
+        }
+      }
+    }
+  }
+
+  std::vector<DevNode2> nodes;
+  std::vector<float> tris;
+  std::vector<float> box;
+  std::vector<Instance> inst_orig, inst_tlas;
+  std::vector<DevNode2> tlas_nodes;
+  ctx->two_level = !ctx->instances_in.empty();
+  uint64_t n_nodes = 0, n_leaves = 0;
+  double sah = 0.0;
+  int depth = 0, tlas_depth = 0;
+  if (!ctx->two_level) {
+    tri_boxes(ctx->pos.data(), 0, n_tris, box);
+    BuildResult b = build_binned_sah(box.data(), n_tris, o.bins, o.max_leaf, threads);
+    append_blas(ctx->pos.data(), 0, b, nodes, tris);
+    // single-level: prim ids are global triangle indices (tri_offset 0)
+    n_nodes = b.nodes.size();
+    n_leaves = b.n_leaves;
+    sah = b.sah_cost;
+    depth = b.depth;
+  } else {
+    std::vector<uint32_t> blas_root(ctx->meshes.size());
+    std::vector<float> mesh_lo(ctx->meshes.size() * 3), mesh_hi(ctx->meshes.size() * 3);
+    for (size_t mi = 0; mi < ctx->meshes.size(); ++mi) {
+      const Mesh& m = ctx->meshes[mi];
+      tri_boxes(ctx->pos.data(), m.tri_offset, m.n_tris, box);
+      BuildResult b = build_binned_sah(box.data(), m.n_tris, o.bins, o.max_leaf, threads);
+      blas_root[mi] = uint32_t(nodes.size());
+      append_blas(ctx->pos.data(), m.tri_offset, b, nodes, tris);
+      n_nodes += b.nodes.size();
+      n_leaves += b.n_leaves;
+      depth = std::max(depth, b.depth);
+      for (int k = 0; k < 3; ++k) {
+        mesh_lo[mi * 3 + k] = b.nodes[0].lo[k];
+        mesh_hi[mi * 3 + k] = b.nodes[0].hi[k];
+      }
+    }
+    const uint32_t ni = uint32_t(ctx->instances_in.size());
+    inst_orig.resize(ni);
+    std::vector<float> ibox(size_t(ni) * 6);
+    for (uint32_t i = 0; i < ni; ++i) {
+      const trt_instance& ti = ctx->instances_in[i];
+      if (ti.mesh >= ctx->meshes.size()) return fail(ctx, TRT_INVALID_ARG, "instance mesh out of range");
+      Instance& in = inst_orig[i];
+      std::memset(&in, 0, sizeof(in));
+      std::memcpy(in.o2w, ti.transform, 48);
+      invert_affine(in.o2w, in.w2o);
+      in.mesh = ti.mesh;
+      in.tri_offset = ctx->meshes[ti.mesh].tri_offset;
+      in.mat_offset = ti.material_offset;
+      in.blas_root = blas_root[ti.mesh];
+      in.id = i;
+      const Mesh& m = ctx->meshes[ti.mesh];
+      for (uint32_t k = 0; k < m.n_tris; ++k)
+        if (ctx->mat_ids[m.tri_offset + k] + ti.material_offset >= n_mats)
+          return fail(ctx, TRT_INVALID_ARG, "instance material id out of range");
+      // world box of the transformed mesh box corners
+      float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+      for (int cc = 0; cc < 8; ++cc) {
+        const V3 p = v3((cc & 1) ? mesh_hi[ti.mesh * 3 + 0] : mesh_lo[ti.mesh * 3 + 0],
+                        (cc & 2) ? mesh_hi[ti.mesh * 3 + 1] : mesh_lo[ti.mesh * 3 + 1],
+                        (cc & 4) ? mesh_hi[ti.mesh * 3 + 2] : mesh_lo[ti.mesh * 3 + 2]);
+        const V3 w = xform_point(in.o2w, p);
+        const float wc[3] = {w.x, w.y, w.z};
+        for (int k = 0; k < 3; ++k) {
+          // widen by a relative ulp margin: the transform is rounded
+          lo[k] = std::min(lo[k], wc[k]);
+          hi[k] = std::max(hi[k], wc[k]);
+        }
+      }
+      for (int k = 0; k < 3; ++k) {
+        const float pad = 1e-5f * std::max(1.f, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+        ibox[size_t(i) * 6 + k] = lo[k] - pad;
+        ibox[size_t(i) * 6 + 3 + k] = hi[k] + pad;
+      }
+    }
+    BuildResult tb = build_binned_sah(ibox.data(), ni, o.bins, o.max_leaf, threads);
+    tlas_nodes = to_device_layout(tb);
+    inst_tlas.resize(ni);
+    for (uint32_t k = 0; k < ni; ++k) inst_tlas[k] = inst_orig[tb.prim_order[k]];
+    tlas_depth = tb.depth;
+    n_nodes += tb.nodes.size();
+  }
+  const int cap = depth + tlas_depth + 2;
+  if (cap > 64) return fail(ctx, TRT_UNSUPPORTED, "BVH too deep for the LDS stack (" + std::to_string(cap) + ")");
+  ctx->stack_cap = cap;
+
+  // lights
+  std::vector<uint32_t> mesh_cnt(ctx->meshes.size());
+  for (size_t i = 0; i < ctx->meshes.size(); ++i) mesh_cnt[i] = ctx->meshes[i].n_tris;
+  LightTable lt = gather_lights(ctx->pos.data(), ctx->mat_ids.data(), n_tris, ctx->materials.data(), n_mats,
+                                ctx->two_level ? inst_orig.data() : nullptr, uint32_t(inst_orig.size()),
+                                mesh_cnt.data());
+  ctx->n_lights = uint32_t(lt.lights.size());
+  ctx->total_power = lt.total_power;
+
+  hipStream_t s = ctx->stream;
+  HIPC(upload(ctx->d_pos, ctx->pos.data(), ctx->pos.size() * 4, s), "upload positions");
+  if (ctx->use_normals) HIPC(upload(ctx->d_nrm, ctx->nrm.data(), ctx->nrm.size() * 4, s), "upload normals");
+  HIPC(upload(ctx->d_mat_ids, ctx->mat_ids.data(), ctx->mat_ids.size() * 4, s), "upload material ids");
+  HIPC(upload(ctx->d_materials, ctx->materials.data(), ctx->materials.size() * sizeof(Material), s), "upload materials");
+  HIPC(upload(ctx->d_nodes, nodes.data(), nodes.size() * sizeof(DevNode2), s), "upload nodes");
+  HIPC(upload(ctx->d_tris, tris.data(), tris.size() * 4, s), "upload triangles");
+  HIPC(upload(ctx->d_lights, lt.lights.data(), lt.lights.size() * sizeof(LightTri), s), "upload lights");
+  HIPC(upload(ctx->d_alias_p, lt.prob.data(), lt.prob.size() * 4, s), "upload alias");
+  HIPC(upload(ctx->d_alias_i, lt.alias.data(), lt.alias.size() * 4, s), "upload alias");
+  if (ctx->two_level) {
+    HIPC(upload(ctx->d_inst, inst_orig.data(), inst_orig.size() * sizeof(Instance), s), "upload instances");
+    HIPC(upload(ctx->d_inst_tlas, inst_tlas.data(), inst_tlas.size() * sizeof(Instance), s), "upload instances");
+    HIPC(upload(ctx->d_tlas, tlas_nodes.data(), tlas_nodes.size() * sizeof(DevNode2), s), "upload tlas");
+  }
+  HIPC(hipStreamSynchronize(s), "upload");
+
+  // persistent grid sizes from occupancy
+  ctx->grid_trace = ctx->num_cus * 4;
+  ctx->grid_shade = ctx->num_cus * 8;
+
+  const auto t1 = std::chrono::steady_clock::now();
+  ctx->info = trt_bvh_info{};
+  ctx->info.nodes = n_nodes;
+  ctx->info.leaves = n_leaves;
+  ctx->info.depth = uint32_t(depth);
+  ctx->info.stack_entries = uint32_t(cap);
+  ctx->info.sah_cost = sah;
+  ctx->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  ctx->info.device_bytes = nodes.size() * sizeof(DevNode2) + tris.size() * 4 + tlas_nodes.size() * sizeof(DevNode2);
+  ctx->info.two_level = ctx->two_level ? 1 : 0;
+  ctx->info.tlas_depth = uint32_t(tlas_depth);
+  ctx->built = true;
+  ctx->sample_count = 0;
+  return TRT_OK;
+}
+
+trt_status trt_bvh_info_get(trt_context* ctx, trt_bvh_info* info) {
+  if (!ctx || !info) return TRT_INVALID_ARG;
+  if (!ctx->built) return fail(ctx, TRT_NOT_BUILT, "scene not built");
+  *info = ctx->info;
+  return TRT_OK;
+}
+
+trt_status trt_set_camera(trt_context* ctx, const trt_camera* c) {
+  if (!ctx || !c) return TRT_INVALID_ARG;
+  if (!finite_all(c->position, 3) || !finite_all(c->target, 3) || !finite_all(c->up, 3) || !(c->fov > 0.f) ||
+      !(c->fov < 3.14159f))
+    return fail(ctx, TRT_INVALID_ARG, "invalid camera");
+  const V3 p = v3(c->position[0], c->position[1], c->position[2]);
+  const V3 t = v3(c->target[0], c->target[1], c->target[2]);
+  const V3 u = v3(c->up[0], c->up[1], c->up[2]);
+  if (length(t - p) == 0.f || length(cross(normalize(t - p), u)) == 0.f)
+    return fail(ctx, TRT_INVALID_ARG, "degenerate camera basis");
+  ctx->cam = make_camera(p, t, u, c->fov, c->aperture, c->focal_dist, c->distortion, c->bokeh_sides, c->bokeh_rotation);
+  ctx->camera_set = true;
+  ctx->sample_count = 0;  // the reference resets accumulation on camera change
+  return TRT_OK;
+}
+
+static trt_status check_render(trt_context* ctx, const trt_render_opts* o) {
+  if (!ctx || !o) return TRT_INVALID_ARG;
+  if (!ctx->built) return fail(ctx, TRT_NOT_BUILT, "scene not built");
+  if (!ctx->camera_set) return fail(ctx, TRT_INVALID_ARG, "camera not set");
+  if (o->width == 0 || o->height == 0 || o->spp == 0 || o->max_depth == 0)
+    return fail(ctx, TRT_INVALID_ARG, "width, height, spp and max_depth must be positive");
+  if (o->max_depth > uint32_t(kMaxDepth - 1)) return fail(ctx, TRT_INVALID_ARG, "max_depth must be < 32");
+  if (!(o->ray_epsilon >= 0.f) || !(o->clamp >= 0.f)) return fail(ctx, TRT_INVALID_ARG, "invalid epsilon/clamp");
+  if (uint64_t(o->width) * o->height * o->spp >= (1ull << 30))
+    return fail(ctx, TRT_INVALID_ARG, "width*height*spp must be below 2^30");
+  return TRT_OK;
+}
+
+trt_status trt_render(trt_context* ctx, const trt_render_opts* o, trt_stats* stats) {
+  trt_status s = check_render(ctx, o);
+  if (s != TRT_OK) return s;
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  return run_frame(ctx, *o, false, true, stats);
+}
+
+trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* o, trt_trace_counts* out) {
+  trt_status s = check_render(ctx, o);
+  if (s != TRT_OK) return s;
+  if (!out) return TRT_INVALID_ARG;
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  trt_stats st;
+  s = run_frame(ctx, *o, true, false, &st);
+  if (s != TRT_OK) return s;
+  unsigned long long c[4];
+  HIPC(hipMemcpy(c, ctx->counters.as<uint32_t>() + kCtrStats, sizeof(c), hipMemcpyDeviceToHost), "read stats");
+  std::memset(out, 0, sizeof(*out));
+  out->closest_rays = st.extend_rays;
+  out->any_rays = st.shadow_rays;
+  out->closest_nodes_per_ray = st.extend_rays ? double(c[0]) / st.extend_rays : 0.0;
+  out->closest_tris_per_ray = st.extend_rays ? double(c[1]) / st.extend_rays : 0.0;
+  out->any_nodes_per_ray = st.shadow_rays ? double(c[2]) / st.shadow_rays : 0.0;
+  out->any_tris_per_ray = st.shadow_rays ? double(c[3]) / st.shadow_rays : 0.0;
+  out->node_bytes = sizeof(DevNode2);
+  out->tri_bytes = 48;
+  return TRT_OK;
+}
+
+trt_status trt_reset(trt_context* ctx) {
+  if (!ctx) return TRT_INVALID_ARG;
+  ctx->sample_count = 0;
+  return TRT_OK;
+}
+
+uint32_t trt_sample_count(const trt_context* ctx) { return ctx ? ctx->sample_count : 0u; }
+
+static trt_status read_pixels(trt_context* ctx, DevBuf& b, float* out) {
+  if (!ctx || !out) return TRT_INVALID_ARG;
+  if (ctx->width == 0) return fail(ctx, TRT_NOT_BUILT, "nothing rendered yet");
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPC(hipMemcpy(out, b.p, size_t(ctx->width) * ctx->height * 16, hipMemcpyDeviceToHost), "read pixels");
+  return TRT_OK;
+}
+trt_status trt_read_accum(trt_context* ctx, float* rgba) { return ctx ? read_pixels(ctx, ctx->accum, rgba) : TRT_INVALID_ARG; }
+trt_status trt_read_color(trt_context* ctx, float* rgba) { return ctx ? read_pixels(ctx, ctx->color, rgba) : TRT_INVALID_ARG; }
+trt_status trt_read_albedo(trt_context* ctx, float* rgba) { return ctx ? read_pixels(ctx, ctx->albedo, rgba) : TRT_INVALID_ARG; }
+trt_status trt_read_normal(trt_context* ctx, float* rgba) { return ctx ? read_pixels(ctx, ctx->normal, rgba) : TRT_INVALID_ARG; }
+
+trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths) {
+  if (!ctx || !rgb) return TRT_INVALID_ARG;
+  if (n_paths != ctx->last_paths || n_paths == 0) return fail(ctx, TRT_INVALID_ARG, "n_paths must equal the last frame's path count");
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  std::vector<float> tmp(n_paths * 4);
+  HIPC(hipMemcpy(tmp.data(), ctx->radiance.p, n_paths * 16, hipMemcpyDeviceToHost), "read radiance");
+  for (uint64_t i = 0; i < n_paths; ++i) {
+    rgb[i * 3 + 0] = tmp[i * 4 + 0];
+    rgb[i * 3 + 1] = tmp[i * 4 + 1];
+    rgb[i * 3 + 2] = tmp[i * 4 + 2];
+  }
+  return TRT_OK;
+}
+
+static trt_status trace_common(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst,
+                               uint8_t* occ, bool any) {
+  if (!ctx || (n && !rays)) return TRT_INVALID_ARG;
+  if (!ctx->built) return fail(ctx, TRT_NOT_BUILT, "scene not built");
+  if (n == 0) return TRT_OK;
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPC(ctx->scratch_rays.ensure(size_t(n) * 32), "alloc rays");
+  HIPC(ctx->scratch_hits.ensure(size_t(n) * 16), "alloc hits");
+  HIPC(ctx->scratch_inst.ensure(size_t(n) * 4), "alloc hits");
+  HIPC(ctx->scratch_occ.ensure(size_t(n) * 4), "alloc hits");
+  HIPC(hipMemcpyAsync(ctx->scratch_rays.p, rays, size_t(n) * 32, hipMemcpyHostToDevice, ctx->stream), "upload rays");
+  uint32_t* ctr = ctx->counters.as<uint32_t>();
+  launch_zero_counters(ctr, ctx->stream);
+  launch_trace(accel_of(ctx), ctx->scratch_rays.as<float4>(), n, ctx->scratch_hits.as<float4>(),
+               ctx->scratch_inst.as<uint32_t>(), ctx->scratch_occ.as<uint32_t>(), any, ctr, ctx->grid_trace,
+               ctx->stream);
+  HIPC(hipGetLastError(), "trace launch");
+  HIPC(hipStreamSynchronize(ctx->stream), "trace");
+  uint32_t overflow = 0;
+  HIPC(hipMemcpy(&overflow, ctr + kCtrOverflow, 4, hipMemcpyDeviceToHost), "read overflow");
+  if (overflow) return fail(ctx, TRT_DEVICE, "traversal stack overflow");
+  if (any) {
+    std::vector<uint32_t> o(n);
+    HIPC(hipMemcpy(o.data(), ctx->scratch_occ.p, size_t(n) * 4, hipMemcpyDeviceToHost), "read occlusion");
+    for (uint32_t i = 0; i < n; ++i) occ[i] = uint8_t(o[i]);
+  } else {
+    HIPC(hipMemcpy(hits, ctx->scratch_hits.p, size_t(n) * 16, hipMemcpyDeviceToHost), "read hits");
+    if (inst) HIPC(hipMemcpy(inst, ctx->scratch_inst.p, size_t(n) * 4, hipMemcpyDeviceToHost), "read hits");
+  }
+  return TRT_OK;
+}
+
+trt_status trt_trace_closest(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst) {
+  if (!hits) return TRT_INVALID_ARG;
+  return trace_common(ctx, rays, n, hits, inst, nullptr, false);
+}
+
+trt_status trt_trace_any(trt_context* ctx, const float* rays, uint32_t n, uint8_t* occluded) {
+  if (!occluded) return TRT_INVALID_ARG;
+  return trace_common(ctx, rays, n, nullptr, nullptr, occluded, true);
+}
+
+trt_status trt_pick(trt_context* ctx, uint32_t width, uint32_t height, uint32_t x, uint32_t y, trt_pick_result* out) {
+  if (!ctx || !out) return TRT_INVALID_ARG;
+  if (!ctx->camera_set) return fail(ctx, TRT_INVALID_ARG, "camera not set");
+  if (x >= width || y >= height) return fail(ctx, TRT_INVALID_ARG, "pixel outside the film");
+  V3 o, d;
+  camera_ray_from(ctx->cam, x, y, width, height, 0.5f, 0.5f, 0.5f, 0.5f, 0.f, o, d);
+  const float ray[8] = {o.x, o.y, o.z, 0.f, d.x, d.y, d.z, 1e30f};
+  float hit[4];
+  uint32_t inst = kNoInstance;
+  trt_status s = trt_trace_closest(ctx, ray, 1, hit, &inst);
+  if (s != TRT_OK) return s;
+  out->origin[0] = o.x; out->origin[1] = o.y; out->origin[2] = o.z;
+  out->direction[0] = d.x; out->direction[1] = d.y; out->direction[2] = d.z;
+  std::memcpy(&out->primitive, &hit[3], 4);
+  out->t = out->primitive == kNoHit ? 1e30f : hit[0];
+  out->instance = inst;
+  return TRT_OK;
+}
+
+}  // extern "C"

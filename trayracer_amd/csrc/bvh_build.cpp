@@ -1,0 +1,307 @@
+// Binned-SAH BVH builder (host side). See bvh_build.h.
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+
+namespace trt {
+namespace {
+
+struct Box {
+  float lo[3], hi[3];
+  void reset() {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::numeric_limits<float>::infinity();
+      hi[k] = -std::numeric_limits<float>::infinity();
+    }
+  }
+  void grow(const Box& b) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], b.lo[k]);
+      hi[k] = std::max(hi[k], b.hi[k]);
+    }
+  }
+  void grow_pt(const float* p) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], p[k]);
+      hi[k] = std::max(hi[k], p[k]);
+    }
+  }
+  double area() const {
+    const double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+    if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+  }
+};
+
+struct Ctx {
+  const float* box;  // 6 per prim
+  std::vector<float> cen;  // 3 per prim
+  std::vector<uint32_t>* order;
+  int bins;
+  int max_leaf;
+  std::atomic<int> threads_left;
+};
+
+struct SubTree {
+  std::vector<BuildNode> nodes;
+  int depth = 0;
+};
+
+Box prim_box(const Ctx& c, uint32_t p) {
+  Box b;
+  std::memcpy(b.lo, c.box + size_t(p) * 6, 12);
+  std::memcpy(b.hi, c.box + size_t(p) * 6 + 3, 12);
+  return b;
+}
+
+// Builds the subtree over order[begin, end) into st.nodes (local indices); returns its
+// root's local index.
+int build_rec(Ctx& c, uint32_t begin, uint32_t end, SubTree& st, int level);
+
+int make_leaf(Ctx& c, uint32_t begin, uint32_t end, const Box& bb, SubTree& st) {
+  BuildNode nd;
+  std::memcpy(nd.lo, bb.lo, 12);
+  std::memcpy(nd.hi, bb.hi, 12);
+  nd.left = nd.right = -1;
+  nd.first = begin;
+  nd.count = end - begin;
+  st.nodes.push_back(nd);
+  (void)c;
+  return int(st.nodes.size()) - 1;
+}
+
+int build_rec(Ctx& c, uint32_t begin, uint32_t end, SubTree& st, int level) {
+  std::vector<uint32_t>& ord = *c.order;
+  const uint32_t n = end - begin;
+  Box bb, cb;
+  bb.reset();
+  cb.reset();
+  for (uint32_t i = begin; i < end; ++i) {
+    bb.grow(prim_box(c, ord[i]));
+    cb.grow_pt(&c.cen[size_t(ord[i]) * 3]);
+  }
+  if (n <= 1) return make_leaf(c, begin, end, bb, st);
+
+  // binned SAH over all three axes
+  const int B = c.bins;
+  double best_cost = std::numeric_limits<double>::infinity();
+  int best_axis = -1, best_split = -1;
+  std::vector<Box> bin_box(B);
+  std::vector<uint32_t> bin_cnt(B);
+  std::vector<double> right_area(B);
+  std::vector<uint32_t> right_cnt(B);
+  for (int axis = 0; axis < 3; ++axis) {
+    const float lo = cb.lo[axis], hi = cb.hi[axis];
+    if (!(hi > lo)) continue;
+    const double scale = double(B) / (double(hi) - double(lo));
+    for (int b = 0; b < B; ++b) {
+      bin_box[b].reset();
+      bin_cnt[b] = 0;
+    }
+    for (uint32_t i = begin; i < end; ++i) {
+      const uint32_t p = ord[i];
+      int b = int((double(c.cen[size_t(p) * 3 + axis]) - lo) * scale);
+      b = std::min(std::max(b, 0), B - 1);
+      bin_box[b].grow(prim_box(c, p));
+      bin_cnt[b]++;
+    }
+    Box acc;
+    acc.reset();
+    uint32_t cnt = 0;
+    for (int b = B - 1; b > 0; --b) {
+      acc.grow(bin_box[b]);
+      cnt += bin_cnt[b];
+      right_area[b] = acc.area();
+      right_cnt[b] = cnt;
+    }
+    acc.reset();
+    cnt = 0;
+    for (int s = 1; s < B; ++s) {  // split between bin s-1 and s
+      acc.grow(bin_box[s - 1]);
+      cnt += bin_cnt[s - 1];
+      if (cnt == 0 || right_cnt[s] == 0) continue;
+      const double cost = acc.area() * cnt + right_area[s] * right_cnt[s];
+      if (cost < best_cost) {
+        best_cost = cost;
+        best_axis = axis;
+        best_split = s;
+      }
+    }
+  }
+  const double parent_area = bb.area();
+  const double leaf_cost = double(n);
+  const double split_cost = parent_area > 0 ? 1.0 + best_cost / parent_area : std::numeric_limits<double>::infinity();
+  if (n <= uint32_t(c.max_leaf) && (best_axis < 0 || leaf_cost <= split_cost)) return make_leaf(c, begin, end, bb, st);
+
+  uint32_t mid;
+  if (best_axis < 0) {
+    mid = begin + n / 2;  // all centroids coincide: split the range in half
+  } else {
+    const float lo = cb.lo[best_axis], hi = cb.hi[best_axis];
+    const double scale = double(B) / (double(hi) - double(lo));
+    auto it = std::partition(ord.begin() + begin, ord.begin() + end, [&](uint32_t p) {
+      int b = int((double(c.cen[size_t(p) * 3 + best_axis]) - lo) * scale);
+      b = std::min(std::max(b, 0), B - 1);
+      return b < best_split;
+    });
+    mid = uint32_t(it - ord.begin());
+    if (mid == begin || mid == end) mid = begin + n / 2;
+  }
+
+  BuildNode nd;
+  std::memcpy(nd.lo, bb.lo, 12);
+  std::memcpy(nd.hi, bb.hi, 12);
+  nd.first = 0;
+  nd.count = 0;
+  st.nodes.push_back(nd);
+  const int me = int(st.nodes.size()) - 1;
+
+  const bool par = n > 65536 && c.threads_left.fetch_sub(1) > 0;
+  if (par) {
+    SubTree right_tree;
+    auto fut = std::async(std::launch::async, [&]() { return build_rec(c, mid, end, right_tree, level + 1); });
+    const int l = build_rec(c, begin, mid, st, level + 1);
+    const int r_local = fut.get();
+    const int off = int(st.nodes.size());
+    for (BuildNode x : right_tree.nodes) {
+      if (x.count == 0) {
+        x.left += off;
+        x.right += off;
+      }
+      st.nodes.push_back(x);
+    }
+    st.nodes[me].left = l;
+    st.nodes[me].right = r_local + off;
+  } else {
+    const int l = build_rec(c, begin, mid, st, level + 1);
+    const int r = build_rec(c, mid, end, st, level + 1);
+    st.nodes[me].left = l;
+    st.nodes[me].right = r;
+  }
+  return me;
+}
+
+void finish_stats(BuildResult& r) {
+  // depth in internal nodes, SAH cost, leaf count (iterative DFS)
+  struct It { int node, d; };
+  std::vector<It> stack{{0, 0}};
+  const double root_area = [&] {
+    Box b;
+    std::memcpy(b.lo, r.nodes[0].lo, 12);
+    std::memcpy(b.hi, r.nodes[0].hi, 12);
+    return b.area();
+  }();
+  r.depth = 0;
+  r.sah_cost = 0;
+  r.n_leaves = 0;
+  while (!stack.empty()) {
+    const It it = stack.back();
+    stack.pop_back();
+    const BuildNode& nd = r.nodes[size_t(it.node)];
+    Box b;
+    std::memcpy(b.lo, nd.lo, 12);
+    std::memcpy(b.hi, nd.hi, 12);
+    const double a = root_area > 0 ? b.area() / root_area : 1.0;
+    if (nd.count > 0 || nd.left < 0) {
+      r.n_leaves++;
+      r.sah_cost += a * nd.count;
+      r.depth = std::max(r.depth, it.d);
+    } else {
+      r.sah_cost += a;
+      stack.push_back({nd.left, it.d + 1});
+      stack.push_back({nd.right, it.d + 1});
+    }
+  }
+}
+
+}  // namespace
+
+BuildResult build_binned_sah(const float* prim_box_data, uint32_t n, int bins, int max_leaf, int n_threads) {
+  BuildResult res;
+  res.prim_order.resize(n);
+  for (uint32_t i = 0; i < n; ++i) res.prim_order[i] = i;
+  if (n == 0) {
+    BuildNode nd{};
+    nd.left = nd.right = -1;
+    nd.first = 0;
+    nd.count = 0;
+    res.nodes.push_back(nd);
+    res.n_leaves = 1;
+    return res;
+  }
+  Ctx c;
+  c.box = prim_box_data;
+  c.cen.resize(size_t(n) * 3);
+  for (uint32_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k)
+      c.cen[size_t(i) * 3 + k] = 0.5f * (prim_box_data[size_t(i) * 6 + k] + prim_box_data[size_t(i) * 6 + 3 + k]);
+  c.order = &res.prim_order;
+  c.bins = std::max(2, bins);
+  c.max_leaf = std::max(1, std::min(7, max_leaf));
+  c.threads_left = std::max(0, n_threads - 1);
+  SubTree st;
+  st.nodes.reserve(size_t(n) * 2 / std::max(1, c.max_leaf / 2) + 16);
+  build_rec(c, 0, n, st, 0);
+  res.nodes = std::move(st.nodes);
+  finish_stats(res);
+  return res;
+}
+
+std::vector<DevNode2> to_device_layout(const BuildResult& b) {
+  std::vector<DevNode2> out;
+  const BuildNode& root = b.nodes[0];
+  auto is_leaf = [&](const BuildNode& nd) { return nd.count > 0 || nd.left < 0; };
+  auto fill_child = [&](DevNode2& d, int slot, const BuildNode& ch, int32_t ref) {
+    if (slot == 0) {
+      d.r0[0] = ch.lo[0]; d.r0[1] = ch.hi[0]; d.r0[2] = ch.lo[1]; d.r0[3] = ch.hi[1];
+      d.r2[0] = ch.lo[2]; d.r2[1] = ch.hi[2];
+    } else {
+      d.r1[0] = ch.lo[0]; d.r1[1] = ch.hi[0]; d.r1[2] = ch.lo[1]; d.r1[3] = ch.hi[1];
+      d.r2[2] = ch.lo[2]; d.r2[3] = ch.hi[2];
+    }
+    d.c[slot] = ref;
+  };
+  if (is_leaf(root)) {
+    DevNode2 d{};
+    fill_child(d, 0, root, leaf_code(root.first, root.count));
+    BuildNode empty{};
+    empty.count = 0;
+    fill_child(d, 1, empty, leaf_code(0, 0));
+    out.push_back(d);
+    return out;
+  }
+  // depth-first numbering of internal nodes, left child first
+  std::vector<int> dev_index(b.nodes.size(), -1);
+  std::vector<int> order;
+  order.reserve(b.nodes.size() / 2 + 1);
+  std::vector<int> stack{0};
+  while (!stack.empty()) {
+    const int i = stack.back();
+    stack.pop_back();
+    dev_index[size_t(i)] = int(order.size());
+    order.push_back(i);
+    const BuildNode& nd = b.nodes[size_t(i)];
+    if (!is_leaf(b.nodes[size_t(nd.right)])) stack.push_back(nd.right);
+    if (!is_leaf(b.nodes[size_t(nd.left)])) stack.push_back(nd.left);
+  }
+  out.resize(order.size());
+  for (size_t k = 0; k < order.size(); ++k) {
+    const BuildNode& nd = b.nodes[size_t(order[k])];
+    DevNode2 d{};
+    const int ch[2] = {nd.left, nd.right};
+    for (int s = 0; s < 2; ++s) {
+      const BuildNode& cn = b.nodes[size_t(ch[s])];
+      const int32_t ref = is_leaf(cn) ? leaf_code(cn.first, cn.count) : dev_index[size_t(ch[s])];
+      fill_child(d, s, cn, ref);
+    }
+    out[k] = d;
+  }
+  return out;
+}
+
+}  // namespace trt

@@ -1,0 +1,51 @@
+// Binned-SAH BVH builder (host) and the device node layout.
+//
+// Replaces the OptiX acceleration-structure builds of the reference (BLAS per model,
+// Resources/Model.cpp:240-306; TLAS over instances, Optix/OptixRenderer.cpp:90-123),
+// which are vendor code. The builder is the textbook binned SAH (Wald 2007, "On fast
+// construction of SAH-based bounding volume hierarchies"): 32 centroid bins per axis,
+// all three axes evaluated, leaves of at most `max_leaf` primitives, subtrees built
+// in parallel on host threads.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace trt {
+
+struct BuildNode {
+  float lo[3], hi[3];
+  int32_t left, right;  // children (internal nodes)
+  uint32_t first, count;  // leaf range in prim_order (count > 0 => leaf)
+};
+
+struct BuildResult {
+  std::vector<BuildNode> nodes;  // nodes[0] = root
+  std::vector<uint32_t> prim_order;
+  int depth = 0;          // max number of internal nodes on a root-to-leaf path
+  double sah_cost = 0.0;  // SAH cost of the tree (traversal 1, intersection 1)
+  uint32_t n_leaves = 0;
+};
+
+// prim_box: 6 floats per primitive (lo xyz, hi xyz).
+BuildResult build_binned_sah(const float* prim_box, uint32_t n, int bins, int max_leaf, int n_threads);
+
+// Device layout: one node stores both children's boxes (64 bytes, four float4 rows):
+//   row0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//   row1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//   row2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//   row3 = (child0, child1, 0, 0) as int32
+// child >= 0: index of an internal node; child < 0: leaf ~((first << 3) | count),
+// first indexing the reordered triangle array, count in [0, 7].
+struct alignas(16) DevNode2 {
+  float r0[4], r1[4], r2[4];
+  int32_t c[4];
+};
+
+inline int32_t leaf_code(uint32_t first, uint32_t count) { return ~static_cast<int32_t>((first << 3) | count); }
+
+// Converts a build tree into the device layout (depth-first order, node 0 = root).
+// Returns the device nodes; a root leaf is wrapped in a node with an empty second child.
+std::vector<DevNode2> to_device_layout(const BuildResult& b);
+
+}  // namespace trt

@@ -1,0 +1,86 @@
+// Device-side data structures of the wavefront renderer (shared by kernels.hip and
+// the host context in api.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bvh_build.h"
+#include "trt/integrator.h"
+
+namespace trt {
+
+// Structure-of-arrays queue of path segments (one slot per live ray). Each field is
+// its own array so that a wave reads/writes 64 consecutive 16-byte records.
+struct RayQueue {
+  float4* o_tmin;   // origin, tmin
+  float4* d_tmax;   // direction, tmax
+  float4* t_pdf;    // throughput rgb, pdf of the sampled direction
+  uint32_t* meta;   // path << 2 | flags
+};
+
+// Closest-hit results, indexed like the ray queue slot that produced them.
+struct HitBuf {
+  float4* thit;     // t, b1, b2, prim (bits)
+  uint32_t* inst;   // instance index (two-level scenes)
+};
+
+// Shadow-ray queue.
+struct ShadowQueue {
+  float4* o_tmin;
+  float4* d_tmax;
+  float4* contrib;  // rgb, path (bits)
+};
+
+// Acceleration structure views.
+struct DevAccel {
+  const DevNode2* nodes;    // single-level BVH, or all BLAS nodes for two-level scenes
+  const float4* tris;       // reordered traversal triangles: 3 float4 per triangle (w of row 0 = prim id)
+  const DevNode2* tlas;     // top level over instances (nullptr for single-level)
+  const Instance* inst;     // instances (two-level)
+  int stack_cap;            // LDS stack entries per lane
+  int two_level;
+};
+
+// Counters in device memory; zeroed at the start of every frame by a kernel.
+// Layout (uint32 indices):
+//   [kCtrExt + d]   rays in the extension queue of depth d (d = 0: primary rays)
+//   [kCtrShadow + d] shadow rays emitted at depth d
+//   [kCtrWorkExt + d], [kCtrWorkShade + d], [kCtrWorkShadow + d]: work-fetch cursors
+//   [kCtrStats + k]  traversal statistics (node visits, triangle tests: 64-bit pairs)
+constexpr int kMaxDepth = 32;
+constexpr int kCtrExt = 0;
+constexpr int kCtrShadow = 32;
+constexpr int kCtrWorkExt = 64;
+constexpr int kCtrWorkShade = 96;
+constexpr int kCtrWorkShadow = 128;
+constexpr int kCtrTrace = 160;  // work cursor of the standalone trace entry points
+constexpr int kCtrOverflow = 161;
+constexpr int kCtrStats = 164;  // 4 x uint64: closest nodes, closest tris, any nodes, any tris
+constexpr int kNumCounters = 192;
+
+struct FrameBuffers {
+  float4* radiance;   // per path (rgb)
+  float4* accum;      // per pixel: rgb sum, sample count
+  float4* color;      // per pixel mean
+  float4* albedo;     // per pixel first-hit albedo (sample 0)
+  float4* normal;     // per pixel first-hit shading normal (xyz), view depth (w)
+};
+
+// Kernel launchers (kernels.hip). All run on `stream`.
+void launch_zero_counters(uint32_t* ctr, hipStream_t stream);
+void launch_raygen(const RenderParams& rp, uint32_t n_paths, RayQueue q, float4* radiance, uint32_t* ctr,
+                   hipStream_t stream);
+void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, const uint32_t* count, uint32_t* work,
+                   uint32_t* ctr, int grid, bool stats, hipStream_t stream);
+void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
+                  HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, const uint32_t* count, uint32_t* work,
+                  uint32_t* next_count, uint32_t* shadow_count, uint32_t depth, int grid, hipStream_t stream);
+void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, const uint32_t* count, uint32_t* work,
+                   uint32_t* ctr, int grid, bool stats, hipStream_t stream);
+void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
+                     hipStream_t stream);
+// standalone traversal of caller rays (test hooks): rays as (o, tmin, d, tmax)
+void launch_trace(const DevAccel& acc, const float4* rays, uint32_t n, float4* hits, uint32_t* hit_inst,
+                  uint32_t* occluded, bool any, uint32_t* ctr, int grid, hipStream_t stream);
+
+}  // namespace trt

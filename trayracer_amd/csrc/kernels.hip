@@ -1,0 +1,490 @@
+// Wavefront path-tracing kernels for gfx950 (CDNA4, wave64).
+//
+// Replaces the reference's device programs: __raygen__ Primary/Secondary/Shadow with
+// the OptiX traversal behind optixTrace (Optix/GPU/optix.cu:225-343), the shading
+// kernel PathTracingKernel (CUDA/GPU/Shade/PathTracing.h:8-175, launched from
+// CUDA/GPU/cuda.cu:22-84), InitCudaCountersKernel and FimalizeFrameKernel
+// (CUDA/GPU/CudaUtilityKernels.h:6-44).
+//
+// Design:
+//  * Extend / Shadow: persistent waves fetch 64 rays per atomic from a queue of
+//    structure-of-arrays ray records; each lane traverses the BVH2 with a per-lane
+//    stack in LDS (lane-interleaved, bank-conflict free) sized to the tree depth.
+//  * Shade: the shared integrator step (trt/integrator.h) per ray; extension and
+//    shadow rays are appended with one atomic per wave (ballot + mbcnt prefix) rather
+//    than one atomic per ray as in the reference (PathTracing.h:135-141, 168-173).
+//  * No host synchronisation between bounces: queue sizes stay in device counters.
+//  * Radiance is accumulated per path (one writer per path per kernel) and reduced
+//    per pixel in a fixed sample order in Finalize, so images are deterministic.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace trt {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t prefix_count(uint64_t mask) {
+  // number of set bits of mask below this lane
+  const uint32_t lo = static_cast<uint32_t>(mask), hi = static_cast<uint32_t>(mask >> 32);
+  return __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+}
+
+// Reserves `cnt` slots at *ctr for the wave (called by all lanes, uniform control flow).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* ctr, uint32_t cnt) {
+  uint32_t base = 0;
+  if (lane_id() == 0 && cnt) base = atomicAdd(ctr, cnt);
+  return __shfl(base, 0);
+}
+
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* work) {
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(work, 64u);
+  return __shfl(base, 0);
+}
+
+__device__ __forceinline__ void wave_add_stat(unsigned long long* dst, uint32_t v) {
+  // sum over the wave, one atomic
+  uint32_t s = v;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane_id() == 0 && s) atomicAdd(dst, static_cast<unsigned long long>(s));
+}
+
+struct TraceOut {
+  float t, b1, b2;
+  uint32_t prim, inst;
+  uint32_t nodes, tests;
+};
+
+__device__ __forceinline__ void load_node(const DevNode2* __restrict__ nodes, int32_t i, float4& r0, float4& r1,
+                                          float4& r2, int4& c) {
+  const float4* p = reinterpret_cast<const float4*>(nodes + i);
+  r0 = p[0];
+  r1 = p[1];
+  r2 = p[2];
+  c = reinterpret_cast<const int4*>(p)[3];
+}
+
+// BVH2 traversal of one bottom-level tree (or the single-level scene) for an
+// (object-space) ray. Stack lives in LDS: stk[k * 64] for entry k. Returns true on the
+// first hit when ANY. Updates tbest/pbest/ibest with the (t, inst, prim) order.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool traverse_bvh(const DevNode2* __restrict__ nodes, const float4* __restrict__ tris,
+                                             int32_t root, V3 o, V3 d, float tmin, uint32_t inst, uint32_t* stk,
+                                             int sp0, int cap, TraceOut& r, uint32_t* overflow) {
+  const RayTri rt = ray_tri_setup(o, d);
+  const V3 inv = safe_inv(d);
+  int sp = sp0;
+  int32_t node = root;
+  for (;;) {
+    if (node >= 0) {
+      float4 r0, r1, r2;
+      int4 c;
+      load_node(nodes, node, r0, r1, r2, c);
+      if (STATS) r.nodes++;
+      float t0, t1;
+      const bool h0 = slab(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, o, inv, tmin, r.t, t0);
+      const bool h1 = slab(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, o, inv, tmin, r.t, t1);
+      if (h0 && h1) {
+        int32_t nearc = c.x, farc = c.y;
+        if (t1 < t0) {
+          nearc = c.y;
+          farc = c.x;
+        }
+        if (sp < cap) {
+          stk[sp * 64] = static_cast<uint32_t>(farc);
+          ++sp;
+        } else {
+          atomicOr(overflow, 1u);
+        }
+        node = nearc;
+        continue;
+      }
+      if (h0) {
+        node = c.x;
+        continue;
+      }
+      if (h1) {
+        node = c.y;
+        continue;
+      }
+    } else {
+      const uint32_t code = ~static_cast<uint32_t>(node);
+      const uint32_t first = code >> 3, cnt = code & 7u;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const float4* tp = tris + static_cast<size_t>(first + k) * 3u;
+        const float4 a = tp[0], b = tp[1], cc = tp[2];
+        if (STATS) r.tests++;
+        TriHit th;
+        if (intersect_triangle(rt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), tmin, r.t, th)) {
+          const uint32_t prim = __float_as_uint(a.w);
+          if (ANY) {
+            r.t = th.t;
+            r.prim = prim;
+            r.inst = inst;
+            return true;
+          }
+          if (th.t < r.t || inst < r.inst || (inst == r.inst && prim < r.prim)) {
+            r.t = th.t;
+            r.b1 = th.b1;
+            r.b2 = th.b2;
+            r.prim = prim;
+            r.inst = inst;
+          }
+        }
+      }
+    }
+    if (sp == sp0) break;
+    --sp;
+    node = static_cast<int32_t>(stk[sp * 64]);
+  }
+  return false;
+}
+
+// Full trace (single or two-level). r.t must hold tmax on entry.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trace_ray(const DevAccel& acc, V3 o, V3 d, float tmin, uint32_t* stk,
+                                          TraceOut& r, uint32_t* overflow) {
+  if (!acc.two_level) return traverse_bvh<ANY, STATS>(acc.nodes, acc.tris, 0, o, d, tmin, kNoInstance, stk, 0,
+                                                      acc.stack_cap, r, overflow);
+  // top level: leaf ranges index the instance array in TLAS build order
+  const V3 inv = safe_inv(d);
+  int sp = 0;
+  int32_t node = 0;
+  const int cap = acc.stack_cap;
+  for (;;) {
+    if (node >= 0) {
+      float4 r0, r1, r2;
+      int4 c;
+      load_node(acc.tlas, node, r0, r1, r2, c);
+      if (STATS) r.nodes++;
+      float t0, t1;
+      const bool h0 = slab(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, o, inv, tmin, r.t, t0);
+      const bool h1 = slab(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, o, inv, tmin, r.t, t1);
+      if (h0 && h1) {
+        int32_t nearc = c.x, farc = c.y;
+        if (t1 < t0) {
+          nearc = c.y;
+          farc = c.x;
+        }
+        if (sp < cap) {
+          stk[sp * 64] = static_cast<uint32_t>(farc);
+          ++sp;
+        } else {
+          atomicOr(overflow, 1u);
+        }
+        node = nearc;
+        continue;
+      }
+      if (h0) {
+        node = c.x;
+        continue;
+      }
+      if (h1) {
+        node = c.y;
+        continue;
+      }
+    } else {
+      const uint32_t code = ~static_cast<uint32_t>(node);
+      const uint32_t first = code >> 3, cnt = code & 7u;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const Instance& in = acc.inst[first + k];
+        const V3 oo = xform_point(in.w2o, o);
+        const V3 od = xform_vector(in.w2o, d);
+        if (traverse_bvh<ANY, STATS>(acc.nodes, acc.tris, static_cast<int32_t>(in.blas_root), oo, od, tmin, in.id,
+                                     stk, sp, cap, r, overflow))
+          return true;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    node = static_cast<int32_t>(stk[sp * 64]);
+  }
+  return false;
+}
+
+// ------------------------------------------------------------------------------------
+__global__ void k_zero_counters(uint32_t* ctr) {
+  const int i = threadIdx.x;
+  if (i < kNumCounters) ctr[i] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen(RenderParams rp, uint32_t n_paths, RayQueue q, float4* radiance,
+                                                    uint32_t* ctr) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  PathState ps;
+  bool alive = false;
+  if (p < n_paths) {
+    alive = raygen(rp, p, ps);
+    radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const uint64_t m = __ballot(alive);
+  const uint32_t base = wave_reserve(&ctr[kCtrExt + 0], static_cast<uint32_t>(__popcll(m)));
+  if (alive) {
+    const uint32_t s = base + prefix_count(m);
+    q.o_tmin[s] = make_float4(ps.o.x, ps.o.y, ps.o.z, rp.eps);
+    q.d_tmax[s] = make_float4(ps.d.x, ps.d.y, ps.d.z, 1e30f);
+    q.t_pdf[s] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.pdf);
+    q.meta[s] = (p << 2) | ps.flags;
+  }
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(kBlock) void k_extend(DevAccel acc, RayQueue q, HitBuf h, const uint32_t* count_p,
+                                                    uint32_t* work, uint32_t* ctr) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = lane_id();
+  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
+  const uint32_t count = *count_p;
+  for (;;) {
+    const uint32_t base = wave_fetch(work);
+    if (base >= count) break;
+    const uint32_t slot = base + lane;
+    TraceOut r;
+    r.nodes = 0;
+    r.tests = 0;
+    if (slot < count) {
+      const float4 a = q.o_tmin[slot];
+      const float4 b = q.d_tmax[slot];
+      r.t = b.w;
+      r.b1 = 0.f;
+      r.b2 = 0.f;
+      r.prim = kNoHit;
+      r.inst = kNoInstance;
+      trace_ray<false, STATS>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
+      h.thit[slot] = make_float4(r.t, r.b1, r.b2, __uint_as_float(r.prim));
+      if (acc.two_level) h.inst[slot] = r.inst;
+    }
+    if (STATS) {
+      wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 0]), r.nodes);
+      wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 2]), r.tests);
+    }
+  }
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevAccel acc, ShadowQueue sq, float4* radiance,
+                                                    const uint32_t* count_p, uint32_t* work, uint32_t* ctr) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = lane_id();
+  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
+  const uint32_t count = *count_p;
+  for (;;) {
+    const uint32_t base = wave_fetch(work);
+    if (base >= count) break;
+    const uint32_t slot = base + lane;
+    TraceOut r;
+    r.nodes = 0;
+    r.tests = 0;
+    if (slot < count) {
+      const float4 a = sq.o_tmin[slot];
+      const float4 b = sq.d_tmax[slot];
+      r.t = b.w;
+      r.prim = kNoHit;
+      r.inst = kNoInstance;
+      const bool occ =
+          trace_ray<true, STATS>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
+      if (!occ) {
+        const float4 c = sq.contrib[slot];
+        const uint32_t path = __float_as_uint(c.w);
+        float4 L = radiance[path];
+        L.x += c.x;
+        L.y += c.y;
+        L.z += c.z;
+        radiance[path] = L;
+      }
+    }
+    if (STATS) {
+      wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 4]), r.nodes);
+      wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 6]), r.tests);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* sampling_mats, RenderParams rp,
+                                                   RayQueue q, HitBuf h, RayQueue qn, ShadowQueue sq,
+                                                   FrameBuffers fb, const uint32_t* count_p, uint32_t* work,
+                                                   uint32_t* next_count, uint32_t* shadow_count, uint32_t depth,
+                                                   uint32_t two_level) {
+  const uint32_t lane = lane_id();
+  const uint32_t count = *count_p;
+  const uint32_t npix = rp.width * rp.height;
+  for (;;) {
+    const uint32_t base = wave_fetch(work);
+    if (base >= count) break;
+    const uint32_t slot = base + lane;
+    ShadeOut out;
+    out.has_ext = 0;
+    out.has_shadow = 0;
+    uint32_t path = 0;
+    if (slot < count) {
+      const float4 a = q.o_tmin[slot];
+      const float4 b = q.d_tmax[slot];
+      const float4 tp = q.t_pdf[slot];
+      const uint32_t meta = q.meta[slot];
+      const float4 hh = h.thit[slot];
+      PathState ps;
+      ps.o = v3(a.x, a.y, a.z);
+      ps.d = v3(b.x, b.y, b.z);
+      ps.T = v3(tp.x, tp.y, tp.z);
+      ps.pdf = tp.w;
+      ps.flags = meta & 3u;
+      ps.path = meta >> 2;
+      path = ps.path;
+      Hit hit;
+      hit.t = hh.x;
+      hit.b1 = hh.y;
+      hit.b2 = hh.z;
+      hit.prim = __float_as_uint(hh.w);
+      hit.inst = two_level ? h.inst[slot] : kNoInstance;
+      shade(sv, sampling_mats, rp, ps, hit, depth, out);
+      if (!is_black(out.L)) {
+        float4 L = fb.radiance[path];
+        L.x += out.L.x;
+        L.y += out.L.y;
+        L.z += out.L.z;
+        fb.radiance[path] = L;
+      }
+      if (depth == 0 && path < npix) {
+        fb.albedo[path] = make_float4(out.albedo.x, out.albedo.y, out.albedo.z, 1.f);
+        fb.normal[path] = make_float4(out.normal.x, out.normal.y, out.normal.z, out.vdepth);
+      }
+    }
+    const bool ext = out.has_ext != 0;
+    const bool shw = out.has_shadow != 0;
+    const uint64_t me = __ballot(ext);
+    const uint64_t ms = __ballot(shw);
+    const uint32_t be = wave_reserve(next_count, static_cast<uint32_t>(__popcll(me)));
+    const uint32_t bs = wave_reserve(shadow_count, static_cast<uint32_t>(__popcll(ms)));
+    if (ext) {
+      const uint32_t s = be + prefix_count(me);
+      qn.o_tmin[s] = make_float4(out.next.o.x, out.next.o.y, out.next.o.z, rp.eps);
+      qn.d_tmax[s] = make_float4(out.next.d.x, out.next.d.y, out.next.d.z, 1e30f);
+      qn.t_pdf[s] = make_float4(out.next.T.x, out.next.T.y, out.next.T.z, out.next.pdf);
+      qn.meta[s] = (path << 2) | out.next.flags;
+    }
+    if (shw) {
+      const uint32_t s = bs + prefix_count(ms);
+      sq.o_tmin[s] = make_float4(out.s_o.x, out.s_o.y, out.s_o.z, rp.eps);
+      sq.d_tmax[s] = make_float4(out.s_d.x, out.s_d.y, out.s_d.z, out.s_tmax);
+      sq.contrib[s] = make_float4(out.s_contrib.x, out.s_contrib.y, out.s_contrib.z, __uint_as_float(path));
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize(RenderParams rp, const float4* radiance, FrameBuffers fb,
+                                                      uint32_t reset) {
+  const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t npix = rp.width * rp.height;
+  if (pix >= npix) return;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  for (uint32_t s = 0; s < rp.spp; ++s) {
+    const float4 L = radiance[static_cast<size_t>(s) * npix + pix];
+    sx += L.x;
+    sy += L.y;
+    sz += L.z;
+  }
+  float4 acc = reset ? make_float4(0.f, 0.f, 0.f, 0.f) : fb.accum[pix];
+  acc.x += sx;
+  acc.y += sy;
+  acc.z += sz;
+  acc.w += static_cast<float>(rp.spp);
+  fb.accum[pix] = acc;
+  const float inv = acc.w > 0.f ? 1.f / acc.w : 0.f;
+  fb.color[pix] = make_float4(acc.x * inv, acc.y * inv, acc.z * inv, 1.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_trace(DevAccel acc, const float4* rays, uint32_t n, float4* hits,
+                                                   uint32_t* hit_inst, uint32_t* occluded, uint32_t any,
+                                                   uint32_t* ctr) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = lane_id();
+  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
+  for (;;) {
+    const uint32_t base = wave_fetch(&ctr[kCtrTrace]);
+    if (base >= n) break;
+    const uint32_t i = base + lane;
+    if (i < n) {
+      const float4 a = rays[2 * static_cast<size_t>(i)];
+      const float4 b = rays[2 * static_cast<size_t>(i) + 1];
+      TraceOut r;
+      r.t = b.w;
+      r.b1 = 0.f;
+      r.b2 = 0.f;
+      r.prim = kNoHit;
+      r.inst = kNoInstance;
+      r.nodes = 0;
+      r.tests = 0;
+      if (any) {
+        const bool occ =
+            trace_ray<true, false>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
+        occluded[i] = occ ? 1u : 0u;
+      } else {
+        trace_ray<false, false>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
+        hits[i] = make_float4(r.t, r.b1, r.b2, __uint_as_float(r.prim));
+        hit_inst[i] = r.inst;
+      }
+    }
+  }
+}
+
+size_t stack_lds_bytes(const DevAccel& acc) { return static_cast<size_t>(acc.stack_cap) * 64u * 4u * kWavesPerBlock; }
+
+}  // namespace
+
+void launch_zero_counters(uint32_t* ctr, hipStream_t stream) {
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(kNumCounters), 0, stream, ctr);
+}
+
+void launch_raygen(const RenderParams& rp, uint32_t n_paths, RayQueue q, float4* radiance, uint32_t* ctr,
+                   hipStream_t stream) {
+  const uint32_t blocks = (n_paths + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_raygen, dim3(blocks), dim3(kBlock), 0, stream, rp, n_paths, q, radiance, ctr);
+}
+
+void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, const uint32_t* count, uint32_t* work,
+                   uint32_t* ctr, int grid, bool stats, hipStream_t stream) {
+  if (stats)
+    hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, count,
+                       work, ctr);
+  else
+    hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, count,
+                       work, ctr);
+}
+
+void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
+                  HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, const uint32_t* count, uint32_t* work,
+                  uint32_t* next_count, uint32_t* shadow_count, uint32_t depth, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, fb, count,
+                     work, next_count, shadow_count, depth, sv.instances ? 1u : 0u);
+}
+
+void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, const uint32_t* count, uint32_t* work,
+                   uint32_t* ctr, int grid, bool stats, hipStream_t stream) {
+  if (stats)
+    hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance,
+                       count, work, ctr);
+  else
+    hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance,
+                       count, work, ctr);
+}
+
+void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
+                     hipStream_t stream) {
+  const uint32_t npix = rp.width * rp.height;
+  hipLaunchKernelGGL(k_finalize, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, rp, radiance, fb,
+                     reset);
+}
+
+void launch_trace(const DevAccel& acc, const float4* rays, uint32_t n, float4* hits, uint32_t* hit_inst,
+                  uint32_t* occluded, bool any, uint32_t* ctr, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, rays, n, hits, hit_inst,
+                     occluded, any ? 1u : 0u, ctr);
+}
+
+}  // namespace trt

@@ -1,0 +1,213 @@
+// One path vertex of the wavefront integrator, shared by the GPU Shade kernel and the
+// host oracle.
+//
+// Flow follows the reference's PathTracingKernel (CUDA/GPU/Shade/PathTracing.h:8-175):
+// miss -> sky (none here), first-hit AOVs, emitter hit (full emission for camera rays,
+// BSDF-sampled hits weighted by MIS against light sampling) and termination at
+// emitters, next-event estimation towards a power-sampled emissive triangle
+// (CudaLights.h:13-124), BSDF sampling of the extension ray, Russian roulette
+// survival probability min(1, max throughput). The reference's quirks are not
+// reproduced (no double-counted primary emission, albedo included in NEE, light
+// points uniformly distributed on the triangle, the last light is selectable,
+// balance-heuristic MIS weights on both strategies, RR from a configurable depth,
+// counter-based RNG keys per (pixel, sample, depth)).
+//
+// Random-number dimensions of vertex stream (pixel, sample, depth):
+//   0,1 light choice (alias column, coin)   2,3 point on the light
+//   4 lobe choice   5,6 BSDF direction   7 Russian roulette
+#pragma once
+#include "bsdf.h"
+#include "camera.h"
+#include "common.h"
+#include "intersect.h"
+#include "rng.h"
+#include "sampling.h"
+#include "scene.h"
+
+namespace trt {
+
+constexpr uint32_t kFlagSpecular = 1u;  // previous bounce was a delta lobe (full emission)
+
+struct alignas(16) RenderParams {
+  CameraParams cam;
+  uint32_t width, height, spp, max_depth;
+  uint32_t rr_depth, seed, sample_offset, flags;
+  float eps;       // ray epsilon (reference KernelSettings::rayEpsilon)
+  float clamp;     // firefly clamp on the max component of a contribution; 0 = off
+  float pad0, pad1;
+};
+
+struct PathState {
+  V3 o, d;      // ray of the next segment
+  V3 T;         // throughput
+  float pdf;    // solid-angle pdf of the BSDF sample that produced d
+  uint32_t flags;
+  uint32_t path;  // path index = sample_in_frame * W * H + pixel
+};
+
+TRT_FI void path_pixel_sample(const RenderParams& rp, uint32_t path, uint32_t& pixel, uint32_t& sample) {
+  const uint32_t np = rp.width * rp.height;
+  pixel = path % np;
+  sample = rp.sample_offset + path / np;
+}
+
+TRT_FI V3 clamp_contrib(V3 c, float mx) {
+  if (mx <= 0.f) return fix_nan(c);
+  const float m = max3(c);
+  return fix_nan(m > mx ? c * (mx / m) : c);
+}
+
+// Camera ray of a path; returns false when the path carries no weight.
+TRT_FI bool raygen(const RenderParams& rp, uint32_t path, PathState& ps) {
+  uint32_t pixel, sample;
+  path_pixel_sample(rp, path, pixel, sample);
+  Pcg32 rng = rng_for(rp.seed, pixel, sample, kCameraDepth);
+  const uint32_t x = pixel % rp.width, y = pixel / rp.width;
+  V3 o, d;
+  const float w = generate_camera_ray(rp.cam, x, y, rp.width, rp.height, rng, o, d);
+  ps.o = o;
+  ps.d = d;
+  ps.T = v3(w);
+  ps.pdf = 1.f;
+  ps.flags = kFlagSpecular;
+  ps.path = path;
+  return w > 0.f;
+}
+
+struct ShadeOut {
+  V3 L;  // emission to add to the path radiance now
+  uint32_t has_shadow;
+  uint32_t has_ext;
+  V3 s_d;
+  float s_tmax;
+  V3 s_contrib;  // added to the path radiance when the shadow ray is unoccluded
+  V3 s_o;
+  PathState next;
+  // first-hit AOVs (valid at depth 0)
+  V3 albedo, normal;
+  float vdepth;
+};
+
+// Solid-angle pdf of light sampling towards an emitter hit at distance t.
+TRT_FI float light_pdf_hit(const SceneView& s, V3 le, float area, float cos_l, float t) {
+  const float pick = light_power(le, area) / s.total_power;
+  return pick * (t * t) / (area * cos_l);
+}
+
+// One vertex: ps = incoming path state, h = closest hit of ray (ps.o, ps.d).
+// sampling_mats may differ from s.materials only in differentiable renders
+// (sampling decisions use it, BSDF values use s.materials).
+TRT_HD inline void shade(const SceneView& s, const Material* sampling_mats, const RenderParams& rp,
+                         const PathState& ps, const Hit& h, uint32_t depth, ShadeOut& out) {
+  out.L = v3(0.f);
+  out.has_shadow = 0;
+  out.has_ext = 0;
+  out.albedo = v3(0.f);
+  out.normal = v3(0.f);
+  out.vdepth = 0.f;
+  if (h.prim == kNoHit) return;  // no sky: the configured scenes are emitter-lit
+
+  const SurfacePoint sp = surface_at(s, h, ps.o, ps.d);
+  const Material& m = s.materials[sp.mat];
+  const Material& ms = sampling_mats[sp.mat];
+  const V3 le = v3(m.emission[0], m.emission[1], m.emission[2]);
+  if (depth == 0) {
+    out.albedo = is_black(le) ? v3(m.base[0], m.base[1], m.base[2]) : le;
+    out.normal = sp.ns;
+    out.vdepth = h.t * dot(ps.d, v3(rp.cam.fwd[0], rp.cam.fwd[1], rp.cam.fwd[2]));
+  }
+
+  if (!is_black(le)) {
+    const float cos_l = -dot(ps.d, sp.ng_raw);
+    if (cos_l > 0.f) {
+      if (depth == 0 || (ps.flags & kFlagSpecular)) {
+        out.L = clamp_contrib(ps.T * le, rp.clamp);
+      } else {
+        const float pl = light_pdf_hit(s, le, sp.area, cos_l, h.t);
+        const float w = ps.pdf / (ps.pdf + pl);
+        out.L = clamp_contrib(ps.T * le * w, rp.clamp);
+      }
+    }
+    return;  // paths end at emitters
+  }
+
+  uint32_t pixel, sample;
+  path_pixel_sample(rp, ps.path, pixel, sample);
+  Pcg32 rng = rng_for(rp.seed, pixel, sample, depth);
+  float u[8];
+  for (int i = 0; i < 8; ++i) u[i] = rng_float(rng);
+
+  const Frame fr = make_frame(sp.ns);
+  const V3 wo = to_local(fr, -ps.d);
+  if (!(wo.z > 0.f)) return;
+  const BsdfParams<float> bp = params_of(m);
+  const BsdfParams<float> bs = params_of(ms);
+
+  // next-event estimation
+  if (s.n_lights > 0 && s.total_power > 0.f) {
+    const uint32_t li = alias_sample(s.alias_prob, s.alias_idx, s.n_lights, u[0], u[1]);
+    const LightTri& L = s.lights[li];
+    float b1, b2;
+    sample_triangle(u[2], u[3], b1, b2);
+    const V3 p0 = v3(L.p0[0], L.p0[1], L.p0[2]);
+    const V3 e1 = v3(L.e1[0], L.e1[1], L.e1[2]);
+    const V3 e2 = v3(L.e2[0], L.e2[1], L.e2[2]);
+    const V3 q = v3(fma(b2, e2.x, fma(b1, e1.x, p0.x)), fma(b2, e2.y, fma(b1, e1.y, p0.y)),
+                    fma(b2, e2.z, fma(b1, e1.z, p0.z)));
+    const V3 dir = q - sp.p;
+    const float dist2 = dot(dir, dir);
+    const float dist = sqrt(dist2);
+    if (dist > 0.f) {
+      const V3 wl = dir * (1.f / dist);
+      const float cos_l = -dot(wl, v3(L.n[0], L.n[1], L.n[2]));
+      if (cos_l > 0.f && dot(wl, sp.ng) > 0.f) {
+        const V3 wi = to_local(fr, wl);
+        if (wi.z > 0.f) {
+          const T3<float> f = bsdf_f(bp, wo, wi);
+          const float pl = (L.power / s.total_power) * dist2 / (L.area * cos_l);
+          const float pb = bsdf_pdf(bs, wo, wi);
+          const float w = pl / (pl + pb);
+          const float k = wi.z * (w / pl);
+          const V3 c = v3(ps.T.x * f.x * L.le[0] * k, ps.T.y * f.y * L.le[1] * k, ps.T.z * f.z * L.le[2] * k);
+          if (!is_black(c)) {
+            out.has_shadow = 1;
+            out.s_o = sp.p;
+            out.s_d = wl;
+            out.s_tmax = dist - 2.f * rp.eps;
+            out.s_contrib = clamp_contrib(c, rp.clamp);
+          }
+        }
+      }
+    }
+  }
+
+  // extension
+  if (depth + 1 < rp.max_depth) {
+    const BsdfSample smp = bsdf_sample(bs, wo, u[4], u[5], u[6]);
+    if (smp.pdf > 0.f) {
+      const V3 wiw = normalize(to_world(fr, smp.wi));
+      if (dot(wiw, sp.ng) > 0.f) {
+        const T3<float> f = bsdf_f(bp, wo, smp.wi);
+        const float k = smp.wi.z / smp.pdf;
+        V3 Tn = v3(ps.T.x * f.x * k, ps.T.y * f.y * k, ps.T.z * f.z * k);
+        bool alive = !is_black(Tn);
+        if (alive && depth >= rp.rr_depth) {
+          const float q = fmin(1.f, max3(Tn));
+          if (u[7] >= q) alive = false;
+          else Tn = Tn * (1.f / q);
+        }
+        if (alive) {
+          out.has_ext = 1;
+          out.next.o = sp.p;
+          out.next.d = wiw;
+          out.next.T = fix_nan(Tn);
+          out.next.pdf = smp.pdf;
+          out.next.flags = 0u;
+          out.next.path = ps.path;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace trt

@@ -1,0 +1,203 @@
+"""Python mirror of the reference's Renderer / Scene façade over the C ABI.
+
+Method names follow Tracer::Renderer (Tracer/Renderer/Renderer.h:45-97): UpdateScene,
+RenderFrame, Reset, SetCamera, PickRay, Settings / SetSettings (KernelSettings),
+Statistics, SampleCount. Device work runs in libtrt.so on the GPU; nothing here
+computes pixels.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import TrtError
+
+
+def _fp(a):
+    return a.ctypes.data_as(_lib.c_float_p)
+
+
+def _u32(a):
+    return a.ctypes.data_as(_lib.c_u32_p)
+
+
+class KernelSettings:
+    """Reference KernelSettings (Tracer/Common/CommonStructs.h:145-153) plus the film
+    size, Russian-roulette start depth and firefly clamp."""
+
+    def __init__(self, width=256, height=256, spp=1, max_depth=4, rr_depth=3, seed=0, ray_epsilon=1e-3, clamp=0.0):
+        self.width, self.height, self.spp = width, height, spp
+        self.max_depth, self.rr_depth, self.seed = max_depth, rr_depth, seed
+        self.ray_epsilon, self.clamp = ray_epsilon, clamp
+
+    def to_c(self):
+        o = _lib.TrtRenderOpts()
+        o.width, o.height, o.spp = self.width, self.height, self.spp
+        o.max_depth, o.rr_depth, o.seed = self.max_depth, self.rr_depth, self.seed
+        o.ray_epsilon, o.clamp = self.ray_epsilon, self.clamp
+        return o
+
+    @classmethod
+    def from_dict(cls, d):
+        return cls(**d)
+
+
+def camera_struct(cam):
+    c = _lib.TrtCamera()
+    c.position[:] = list(cam["position"])
+    c.target[:] = list(cam["target"])
+    c.up[:] = list(cam["up"])
+    c.fov = cam["fov"]
+    c.aperture = cam.get("aperture", 0.0)
+    c.focal_dist = cam.get("focal_dist", 1.0)
+    c.distortion = cam.get("distortion", 0.0)
+    c.bokeh_sides = int(cam.get("bokeh_sides", 0))
+    c.bokeh_rotation = cam.get("bokeh_rotation", 0.0)
+    return c
+
+
+class Renderer:
+    """Owns one trt context on `device`."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        self.ctx = ctypes.c_void_p()
+        st = self.lib.trt_create(device, ctypes.byref(self.ctx))
+        if st != 0:
+            raise TrtError(st, "trt_create failed (no usable HIP device?)")
+        self.settings = KernelSettings()
+        self.last_stats = None
+
+    def close(self):
+        if self.ctx:
+            self.lib.trt_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise TrtError(st, self.lib.trt_last_error(self.ctx).decode())
+
+    # -- scene -----------------------------------------------------------------------
+    def UpdateScene(self, scene, bins=32, max_leaf=4, threads=0):
+        """Uploads a scene dict (see trayracer_amd.scenes) and builds its BVH(s)."""
+        lib, ctx = self.lib, self.ctx
+        self._check(lib.trt_clear_scene(ctx))
+        mats = np.ascontiguousarray(scene["materials"], np.float32)
+        cm = (_lib.TrtMaterial * len(mats))()
+        ctypes.memmove(cm, mats.ctypes.data, mats.nbytes)
+        self._check(lib.trt_set_materials(ctx, cm, len(mats)))
+        pos = np.ascontiguousarray(scene["positions"], np.float32)
+        nrm = scene.get("normals")
+        nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32)
+        mid = np.ascontiguousarray(scene["mat_ids"], np.uint32)
+        for off, cnt in scene["meshes"]:
+            mesh_id = ctypes.c_uint32()
+            self._check(lib.trt_add_mesh(ctx, _fp(pos[off:off + cnt]), None if nrm is None else _fp(nrm[off:off + cnt]),
+                                         _u32(mid[off:off + cnt]), cnt, ctypes.byref(mesh_id)))
+        inst = scene.get("instances")
+        if inst is not None:
+            inst = np.ascontiguousarray(inst, np.float32)
+            ci = (_lib.TrtInstance * len(inst))()
+            ctypes.memmove(ci, inst.ctypes.data, inst.nbytes)
+            self._check(lib.trt_set_instances(ctx, ci, len(inst)))
+        bo = _lib.TrtBuildOpts(bins, max_leaf, threads)
+        self._check(lib.trt_build(ctx, ctypes.byref(bo)))
+        if "camera" in scene:
+            self.SetCamera(scene["camera"])
+        if "render" in scene:
+            self.SetSettings(KernelSettings.from_dict(scene["render"]))
+
+    def bvh_info(self):
+        info = _lib.TrtBvhInfo()
+        self._check(self.lib.trt_bvh_info_get(self.ctx, ctypes.byref(info)))
+        return info.as_dict()
+
+    # -- camera / settings -----------------------------------------------------------
+    def SetCamera(self, cam):
+        c = camera_struct(cam)
+        self._check(self.lib.trt_set_camera(self.ctx, ctypes.byref(c)))
+
+    def Settings(self):
+        return self.settings
+
+    def SetSettings(self, settings):
+        self.settings = settings
+        self.Reset()
+
+    # -- rendering -------------------------------------------------------------------
+    def RenderFrame(self):
+        st = _lib.TrtStats()
+        o = self.settings.to_c()
+        self._check(self.lib.trt_render(self.ctx, ctypes.byref(o), ctypes.byref(st)))
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def Reset(self):
+        self._check(self.lib.trt_reset(self.ctx))
+
+    def SampleCount(self):
+        return int(self.lib.trt_sample_count(self.ctx))
+
+    def Statistics(self):
+        return self.last_stats
+
+    def measure_traversal(self):
+        tc = _lib.TrtTraceCounts()
+        o = self.settings.to_c()
+        self._check(self.lib.trt_measure_traversal(self.ctx, ctypes.byref(o), ctypes.byref(tc)))
+        return tc.as_dict()
+
+    # -- outputs ---------------------------------------------------------------------
+    def _pixels(self, fn):
+        s = self.settings
+        out = np.empty((s.height, s.width, 4), np.float32)
+        self._check(fn(self.ctx, _fp(out)))
+        return out
+
+    def color(self):
+        return self._pixels(self.lib.trt_read_color)
+
+    def accum(self):
+        return self._pixels(self.lib.trt_read_accum)
+
+    def albedo(self):
+        return self._pixels(self.lib.trt_read_albedo)
+
+    def normal(self):
+        return self._pixels(self.lib.trt_read_normal)
+
+    def path_radiance(self):
+        s = self.settings
+        n = s.width * s.height * s.spp
+        out = np.empty((n, 3), np.float32)
+        self._check(self.lib.trt_read_path_radiance(self.ctx, _fp(out), n))
+        return out
+
+    # -- ray queries -----------------------------------------------------------------
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.empty((n, 4), np.float32)
+        inst = np.empty(n, np.uint32)
+        self._check(self.lib.trt_trace_closest(self.ctx, _fp(rays), n, _fp(hits), _u32(inst)))
+        return hits, inst
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        occ = np.empty(n, np.uint8)
+        self._check(self.lib.trt_trace_any(self.ctx, _fp(rays), n, occ.ctypes.data_as(_lib.c_u8_p)))
+        return occ
+
+    def PickRay(self, x, y):
+        r = _lib.TrtPickResult()
+        s = self.settings
+        self._check(self.lib.trt_pick(self.ctx, s.width, s.height, x, y, ctypes.byref(r)))
+        return dict(origin=tuple(r.origin), direction=tuple(r.direction), t=r.t, instance=r.instance,
+                    primitive=r.primitive)
