@@ -44,7 +44,7 @@ def test_trace_cornell_exact(rend):
     hb, ib = orc.trace_closest(rays, brute=True)
     _check_hits(hr, ir, hb, ib, "oracle bvh vs brute")
     _check_hits(hg, ig, hr, ir, "gpu vs oracle")
-    assert (prim_of(hg) != 0xFFFFFFFF).mean() > 0.99  # rays start inside the closed-ish box
+    assert (prim_of(hg) != 0xFFFFFFFF).mean() > 0.7  # the box is open at z = 0
     occ_g = rend.trace_any(rays[:5000].copy())
     sr = rays[:5000].copy()
     sr[:, 7] = np.random.default_rng(3).uniform(0.05, 1.5, 5000)
