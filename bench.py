@@ -29,7 +29,20 @@ def parse():
     ap.add_argument("--workload", default="soup1m", choices=["soup1m", "cornell", "instanced"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calibrate():
+    import torch
+    n = 1 << 28  # 1 GiB of float32
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
 
 
 def make_scene(name):
@@ -64,6 +77,9 @@ def cpu_baseline(scene, threads):
 
 def main():
     args = parse()
+    if args.copy_calibrate:
+        copy_calibrate()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: every configuration runs on one MI355X; "
                                    "multi-GPU runs are not part of this build (--gpus must be 1)"}))
@@ -97,9 +113,15 @@ def main():
     bytes_per_ext = ray_io + tc["closest_nodes_per_ray"] * tc["node_bytes"] + tc["closest_tris_per_ray"] * tc["tri_bytes"]
     ext_bytes = bytes_per_ext * ext_rays
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "k_extend_soup1m_traffic.json")
+    if args.workload == "soup1m" and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = round(json.load(f)["traffic_bytes_per_launch"])
+        traffic_src = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/k_extend_soup1m_hbm_traffic.md)"
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
         "kernel": "k_extend", "avg_launch_ms": round(ext_ms / max(1, launches), 4),
         "bytes_per_ray": round(bytes_per_ext, 1), "nodes_per_ray": round(tc["closest_nodes_per_ray"], 2),
         "tris_per_ray": round(tc["closest_tris_per_ray"], 2),

@@ -1,8 +1,7 @@
 /* trt — MI355X (gfx950) wavefront path tracer, C ABI.
  *
  * One opaque context owns a HIP stream and every device buffer. Scene data is
- * copied in from caller host arrays; results are copied out to caller host arrays
- * (the *_device variants take device pointers and run on the context's stream).
+ * copied in from caller host arrays; results are copied out to caller host arrays.
  * Calls are synchronous, never abort, and report failures through trt_status plus
  * trt_last_error(). A context is not thread-safe (callers serialise access).
  *
@@ -199,6 +198,18 @@ typedef struct trt_trace_counts {
   uint32_t tri_bytes;
 } trt_trace_counts;
 trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* opts, trt_trace_counts* out);
+
+/* Differentiable rendering. No counterpart in the reference (its PathTracingKernel,
+ * Tracer/CUDA/GPU/Shade/PathTracing.h:8-175, has no adjoint): the adjoint of one frame
+ * by path replay. Parameters per material: base_color rgb, roughness, metallic
+ * (5 floats). which: 1 = values used for BSDF evaluation, 2 = values used for sampling
+ * decisions and MIS weights (held fixed for finite differences), 3 = both. */
+trt_status trt_set_material_params(trt_context* ctx, const float* params, uint32_t n_materials, uint32_t which);
+/* Gradient of a loss with respect to the evaluation parameters of every material for
+ * the last frame rendered by trt_render (same options): dL_dimage = dLoss/d(frame mean
+ * image), 3 floats per pixel; out dL_dparams = 5 floats per material. Records are reduced
+ * in a fixed order (no float atomics): repeated calls return identical bits. */
+trt_status trt_backward(trt_context* ctx, const trt_render_opts* opts, const float* dL_dimage, float* dL_dparams);
 
 #ifdef __cplusplus
 }

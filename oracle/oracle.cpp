@@ -142,6 +142,7 @@ struct OScene {
   std::vector<double> inst_box;
   LightTable lt;
   SceneView view;
+  std::vector<Material> samp;  // sampling materials (differentiable renders)
 };
 
 void tri_box_d(const float* p, double* out) {
@@ -376,7 +377,22 @@ void* orc_scene_create(const float* pos, const float* nrm, const uint32_t* mat_i
   v.total_power = s->lt.total_power;
   v.n_tris = n_tris;
   v.n_materials = n_mats;
+  s->samp = s->mats;
   return s;
+}
+
+// params: 5 floats per material (base rgb, roughness, metallic); which: 1 eval, 2 sampling, 3 both
+void orc_set_material_params(void* p, const float* params, uint32_t n, uint32_t which) {
+  OScene* s = static_cast<OScene*>(p);
+  for (uint32_t i = 0; i < n && i < s->mats.size(); ++i) {
+    for (int pass = 0; pass < 2; ++pass) {
+      if (!(which & (1u << pass))) continue;
+      Material& m = pass == 0 ? s->mats[i] : s->samp[i];
+      for (int k = 0; k < 3; ++k) m.base[k] = params[size_t(i) * 5 + k];
+      m.roughness = params[size_t(i) * 5 + 3];
+      m.metallic = params[size_t(i) * 5 + 4];
+    }
+  }
 }
 
 void orc_scene_destroy(void* p) { delete static_cast<OScene*>(p); }
@@ -460,7 +476,7 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
         trace(*s, ps.o, ps.d, rp.eps, 1e30f, b, false, false);
         Hit h{b.t, b.b1, b.b2, b.prim, b.inst};
         ShadeOut out;
-        shade(s->view, s->view.materials, rp, ps, h, d, out);
+        shade(s->view, s->samp.data(), rp, ps, h, d, out);
         if (!is_black(out.L)) acc = acc + out.L;
         if (d == 0 && path < npix && aov) {
           float* a = aov + size_t(path) * 8;
@@ -502,6 +518,125 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
       if (image_f32) image_f32[px * 3 + k] = sf[k] * invs;
       if (image_f64) image_f64[px * 3 + k] = sd[k] / double(rp.spp);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Adjoint: path-replay backpropagation per path, gradient sums in FP64.
+// dldi: dLoss/d(frame mean image), 3 floats per pixel; grad: 5 doubles per material.
+// Each path is first rendered forward (its total radiance), then replayed with the
+// same random numbers; per vertex the record (NEE term if unoccluded, plus the
+// continuation term once the radiance that arrived through the sampled direction is
+// known) is formed in FP32 exactly as the device does, and added to an FP64 sum.
+void orc_backward(void* p, const float* cam, const uint32_t* uopts, const float* fopts, const float* dldi,
+                  double* grad, int threads) {
+  const OScene* s = static_cast<OScene*>(p);
+  RenderParams rp{};
+  rp.cam = make_camera(v3(cam[0], cam[1], cam[2]), v3(cam[3], cam[4], cam[5]), v3(cam[6], cam[7], cam[8]), cam[9],
+                       cam[10], cam[11], cam[12], int(cam[13]), cam[14]);
+  rp.width = uopts[0];
+  rp.height = uopts[1];
+  rp.spp = uopts[2];
+  rp.max_depth = uopts[3];
+  rp.rr_depth = uopts[4];
+  rp.seed = uopts[5];
+  rp.sample_offset = uopts[6];
+  rp.eps = fopts[0];
+  rp.clamp = fopts[1];
+  const uint64_t npix = uint64_t(rp.width) * rp.height;
+  const uint64_t n = npix * rp.spp;
+  const uint32_t D = rp.max_depth;
+  const size_t P = s->mats.size() * kNumParams;
+  const int T = hw_threads(threads);
+  std::vector<std::vector<double>> acc(size_t(T), std::vector<double>(P, 0.0));
+  std::atomic<uint64_t> next{0};
+  const float inv_spp = 1.f / float(rp.spp);
+  auto work = [&](int tid) {
+    std::vector<double>& g = acc[size_t(tid)];
+    std::vector<float> rec(size_t(D) * kNumParams);
+    std::vector<uint32_t> key(D);
+    for (;;) {
+      const uint64_t b = next.fetch_add(256);
+      if (b >= n) break;
+      for (uint64_t pi = b; pi < std::min<uint64_t>(n, b + 256); ++pi) {
+        const uint32_t path = uint32_t(pi);
+        const uint32_t pix = uint32_t(pi % npix);
+        const V3 delta = v3(dldi[pix * 3 + 0] * inv_spp, dldi[pix * 3 + 1] * inv_spp, dldi[pix * 3 + 2] * inv_spp);
+        // forward: total radiance of the path
+        PathState ps;
+        V3 L = v3(0.f);
+        if (raygen(rp, path, ps)) {
+          for (uint32_t d = 0; d < D; ++d) {
+            Best bb;
+            trace(*s, ps.o, ps.d, rp.eps, 1e30f, bb, false, false);
+            Hit h{bb.t, bb.b1, bb.b2, bb.prim, bb.inst};
+            ShadeOut out;
+            shade(s->view, s->samp.data(), rp, ps, h, d, out);
+            if (!is_black(out.L)) L = L + out.L;
+            if (out.has_shadow) {
+              Best sb;
+              if (!trace(*s, out.s_o, out.s_d, rp.eps, out.s_tmax, sb, true, false)) L = L + out.s_contrib;
+            }
+            if (!out.has_ext) break;
+            ps = out.next;
+          }
+        } else {
+          continue;
+        }
+        // replay
+        raygen(rp, path, ps);
+        V3 lr = L;
+        AdjOut prev;
+        bool has_prev = false;
+        for (uint32_t d = 0; d < D; ++d) key[d] = kNoHit;
+        for (uint32_t d = 0; d < D; ++d) {
+          Best bb;
+          trace(*s, ps.o, ps.d, rp.eps, 1e30f, bb, false, false);
+          Hit h{bb.t, bb.b1, bb.b2, bb.prim, bb.inst};
+          if (d > 0 && has_prev) {
+            float gg[kNumParams];
+            continuation_grad(prev, delta, lr, gg);
+            for (int k = 0; k < kNumParams; ++k) rec[(d - 1) * kNumParams + k] += gg[k];
+          }
+          ShadeOut out;
+          AdjOut adj;
+          shade_adjoint(s->view, s->samp.data(), rp, ps, h, d, delta, out, adj);
+          if (!is_black(out.L)) {
+            lr.x -= out.L.x;
+            lr.y -= out.L.y;
+            lr.z -= out.L.z;
+          }
+          if (adj.mat != kNoHit) {
+            key[d] = adj.mat;
+            for (int k = 0; k < kNumParams; ++k) rec[d * kNumParams + k] = 0.f;
+          }
+          if (out.has_shadow) {
+            Best sb;
+            if (!trace(*s, out.s_o, out.s_d, rp.eps, out.s_tmax, sb, true, false)) {
+              lr.x -= out.s_contrib.x;
+              lr.y -= out.s_contrib.y;
+              lr.z -= out.s_contrib.z;
+              for (int k = 0; k < kNumParams; ++k) rec[d * kNumParams + k] += adj.nee_g[k];
+            }
+          }
+          has_prev = out.has_ext != 0;
+          prev = adj;
+          if (!out.has_ext) break;
+          ps = out.next;
+        }
+        for (uint32_t d = 0; d < D; ++d)
+          if (key[d] != kNoHit)
+            for (int k = 0; k < kNumParams; ++k) g[size_t(key[d]) * kNumParams + k] += double(rec[d * kNumParams + k]);
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t) th.emplace_back(work, t);
+  for (auto& x : th) x.join();
+  for (size_t i = 0; i < P; ++i) {
+    double t = 0.0;
+    for (int k = 0; k < T; ++k) t += acc[size_t(k)][i];
+    grad[i] = t;
   }
 }
 

@@ -24,6 +24,8 @@ _PROTO = [
     ("orc_trace_closest", None, [_P, _f, ctypes.c_uint32, _f, _u, ctypes.c_int, ctypes.c_int]),
     ("orc_trace_any", None, [_P, _f, ctypes.c_uint32, _u8, ctypes.c_int, ctypes.c_int]),
     ("orc_render", None, [_P, _f, _u, _f, _f, _f, _d, _f, _u64, ctypes.c_int]),
+    ("orc_set_material_params", None, [_P, _f, ctypes.c_uint32, ctypes.c_uint32]),
+    ("orc_backward", None, [_P, _f, _u, _f, _f, _d, ctypes.c_int]),
     ("orc_pcg32", None, [ctypes.c_uint64, ctypes.c_uint64, _u, ctypes.c_uint32]),
     ("orc_rng_floats", None, [ctypes.c_uint32] * 4 + [_f, ctypes.c_uint32]),
     ("orc_sincos_turns", None, [_f, _f, _f, ctypes.c_uint32]),
@@ -155,6 +157,26 @@ class Oracle:
         self.lib.orc_render(self.handle, fp(cam), up(uo), fp(fo), fp(paths), fp(img32),
                             img64.ctypes.data_as(_d), fp(aov), counts.ctypes.data_as(_u64), threads)
         return dict(path_rgb=paths, image_f32=img32, image_f64=img64, aov=aov, counts=counts.reshape(2, D))
+
+
+    def set_material_params(self, params, which=3):
+        params = np.ascontiguousarray(params, np.float32).reshape(-1, 5)
+        self.lib.orc_set_material_params(self.handle, fp(params), params.shape[0], which)
+
+    def backward(self, dl_dimage, render=None, camera=None, sample_offset=0, threads=0):
+        """FP64-accumulated gradient [n_materials, 5] of the frame rendered with `render`."""
+        r = dict(self.scene["render"])
+        if render:
+            r.update(render)
+        cam = cam_array(camera or self.scene["camera"])
+        uo = np.array([r["width"], r["height"], r["spp"], r["max_depth"], r["rr_depth"], r["seed"], sample_offset],
+                      np.uint32)
+        fo = np.array([r["ray_epsilon"], r["clamp"]], np.float32)
+        g = np.ascontiguousarray(dl_dimage, np.float32).reshape(-1)
+        nm = np.asarray(self.scene["materials"]).shape[0]
+        out = np.zeros(nm * 5, np.float64)
+        self.lib.orc_backward(self.handle, fp(cam), up(uo), fp(fo), fp(g), out.ctypes.data_as(_d), threads)
+        return out.reshape(nm, 5)
 
 
 # --- unit hooks ------------------------------------------------------------------------

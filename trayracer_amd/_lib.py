@@ -119,6 +119,8 @@ PROTOTYPES = [
     ("trt_pick", S, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                      ctypes.POINTER(TrtPickResult)]),
     ("trt_measure_traversal", S, [P, ctypes.POINTER(TrtRenderOpts), ctypes.POINTER(TrtTraceCounts)]),
+    ("trt_set_material_params", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32]),
+    ("trt_backward", S, [P, ctypes.POINTER(TrtRenderOpts), c_float_p, c_float_p]),
 ]
 
 STATUS = {0: "TRT_OK", 1: "TRT_INVALID_ARG", 2: "TRT_OUT_OF_MEMORY", 3: "TRT_DEVICE", 4: "TRT_NOT_BUILT",

@@ -106,6 +106,14 @@ struct trt_context {
 
   hipEvent_t ev[4 + 4 * kMaxDepth] = {};
   int grid_trace = 0, grid_shade = 0;
+
+  // differentiable rendering
+  std::vector<Material> sampling_materials;  // empty = same as materials
+  DevBuf d_mat_sampling;
+  bool has_frame = false;
+  trt_render_opts last_opts{};
+  uint32_t last_sample_offset = 0;
+  DevBuf a_lrem, a_pend, a_key, a_g, a_sga, a_sgb, a_dldi, a_partial, a_out;
 };
 
 namespace {
@@ -259,6 +267,10 @@ SceneView view_of(const trt_context* ctx) {
   return s;
 }
 
+const Material* sampling_mats(const trt_context* ctx) {
+  return ctx->sampling_materials.empty() ? ctx->d_materials.as<Material>() : ctx->d_mat_sampling.as<Material>();
+}
+
 RenderParams params_of_opts(const trt_context* ctx, const trt_render_opts& o) {
   RenderParams rp{};
   rp.cam = ctx->cam;
@@ -310,7 +322,7 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
     const int cur = int(d & 1u), nxt = cur ^ 1;
     launch_extend(acc, q[cur], hb, &ctr[kCtrExt + d], &ctr[kCtrWorkExt + d], ctr, ctx->grid_trace, stats, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 0], stm), "event");
-    launch_shade(sv, ctx->d_materials.as<Material>(), rp, q[cur], hb, q[nxt], sq, fb, &ctr[kCtrExt + d],
+    launch_shade(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq, fb, &ctr[kCtrExt + d],
                  &ctr[kCtrWorkShade + d], &ctr[kCtrExt + d + 1], &ctr[kCtrShadow + d], d, ctx->grid_shade, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 1], stm), "event");
     launch_shadow(acc, sq, fb.radiance, &ctr[kCtrShadow + d], &ctr[kCtrWorkShadow + d], ctr, ctx->grid_trace, stats,
@@ -324,8 +336,11 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
 
   uint32_t hc[kNumCounters];
   HIPC(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost), "read counters");
-  if (accumulate) ctx->sample_count += o.spp;
   ctx->last_paths = n;
+  ctx->has_frame = true;
+  ctx->last_opts = o;
+  ctx->last_sample_offset = rp.sample_offset;
+  if (accumulate) ctx->sample_count += o.spp;
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->paths = n;
@@ -433,6 +448,7 @@ trt_status trt_set_materials(trt_context* ctx, const trt_material* mats, uint32_
     m[i] = to_material(mats[i]);
   }
   ctx->materials = std::move(m);
+  ctx->sampling_materials.clear();
   ctx->built = false;
   return TRT_OK;
 }
@@ -617,8 +633,8 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   }
   HIPC(hipStreamSynchronize(s), "upload");
 
-  // persistent grid sizes from occupancy
-  ctx->grid_trace = ctx->num_cus * 4;
+  // persistent grid sizes from occupancy (every resident slot gets a wave)
+  ctx->grid_trace = ctx->num_cus * trace_blocks_per_cu(accel_of(ctx));
   ctx->grid_shade = ctx->num_cus * 8;
 
   const auto t1 = std::chrono::steady_clock::now();
@@ -794,6 +810,107 @@ trt_status trt_pick(trt_context* ctx, uint32_t width, uint32_t height, uint32_t 
   std::memcpy(&out->primitive, &hit[3], 4);
   out->t = out->primitive == kNoHit ? 1e30f : hit[0];
   out->instance = inst;
+  return TRT_OK;
+}
+
+
+trt_status trt_set_material_params(trt_context* ctx, const float* params, uint32_t n, uint32_t which) {
+  if (!ctx || (n && !params)) return TRT_INVALID_ARG;
+  if (!ctx->built) return fail(ctx, TRT_NOT_BUILT, "scene not built");
+  if (n != ctx->materials.size()) return fail(ctx, TRT_INVALID_ARG, "n must equal the number of materials");
+  if (which < 1 || which > 3) return fail(ctx, TRT_INVALID_ARG, "which must be 1 (evaluation), 2 (sampling) or 3");
+  if (!finite_all(params, size_t(n) * 5)) return fail(ctx, TRT_INVALID_ARG, "non-finite parameter");
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  if (ctx->sampling_materials.empty()) ctx->sampling_materials = ctx->materials;
+  auto apply = [&](std::vector<Material>& mats) {
+    for (uint32_t i = 0; i < n; ++i) {
+      for (int k = 0; k < 3; ++k) mats[i].base[k] = params[size_t(i) * 5 + k];
+      mats[i].roughness = params[size_t(i) * 5 + 3];
+      mats[i].metallic = params[size_t(i) * 5 + 4];
+    }
+  };
+  if (which & 1) apply(ctx->materials);
+  if (which & 2) apply(ctx->sampling_materials);
+  HIPC(upload(ctx->d_materials, ctx->materials.data(), ctx->materials.size() * sizeof(Material), ctx->stream),
+       "upload materials");
+  HIPC(upload(ctx->d_mat_sampling, ctx->sampling_materials.data(), ctx->sampling_materials.size() * sizeof(Material),
+              ctx->stream),
+       "upload sampling materials");
+  HIPC(hipStreamSynchronize(ctx->stream), "upload materials");
+  ctx->sample_count = 0;
+  return TRT_OK;
+}
+
+trt_status trt_backward(trt_context* ctx, const trt_render_opts* o, const float* dldi, float* grad) {
+  trt_status s = check_render(ctx, o);
+  if (s != TRT_OK) return s;
+  if (!dldi || !grad) return TRT_INVALID_ARG;
+  const trt_render_opts& L = ctx->last_opts;
+  if (!ctx->has_frame || L.width != o->width || L.height != o->height || L.spp != o->spp ||
+      L.max_depth != o->max_depth || L.rr_depth != o->rr_depth || L.seed != o->seed ||
+      L.ray_epsilon != o->ray_epsilon || L.clamp != o->clamp)
+    return fail(ctx, TRT_INVALID_ARG, "trt_backward replays the last trt_render: call it with the same options");
+  if (o->clamp != 0.f) return fail(ctx, TRT_UNSUPPORTED, "the adjoint needs clamp = 0");
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  const uint32_t W = o->width, H = o->height, D = std::min<uint32_t>(o->max_depth, kMaxDepth);
+  const uint64_t n = uint64_t(W) * H * o->spp;
+  const uint64_t R = n * D;
+  const uint32_t nm = uint32_t(ctx->materials.size());
+  HIPC(ctx->a_lrem.ensure(n * 16), "alloc adjoint");
+  HIPC(ctx->a_pend.ensure(n * 64), "alloc adjoint");
+  HIPC(ctx->a_key.ensure(R * 4), "alloc adjoint");
+  HIPC(ctx->a_g.ensure(R * 4 * kNumParams), "alloc adjoint");
+  HIPC(ctx->a_sga.ensure(n * 16), "alloc adjoint");
+  HIPC(ctx->a_sgb.ensure(n * 4), "alloc adjoint");
+  HIPC(ctx->a_dldi.ensure(uint64_t(W) * H * 12), "alloc adjoint");
+  HIPC(ctx->a_partial.ensure(size_t(kGradBlocks) * kGradMatsPerPass * kNumParams * 4), "alloc adjoint");
+  HIPC(ctx->a_out.ensure(size_t(nm) * kNumParams * 4 + 16), "alloc adjoint");
+  HIPC(hipMemcpyAsync(ctx->a_dldi.p, dldi, uint64_t(W) * H * 12, hipMemcpyHostToDevice, ctx->stream), "upload dL/dI");
+
+  RenderParams rp = params_of_opts(ctx, *o);
+  rp.sample_offset = ctx->last_sample_offset;
+  const DevAccel acc = accel_of(ctx);
+  const SceneView sv = view_of(ctx);
+  uint32_t* ctr = ctx->counters.as<uint32_t>();
+  AdjBuffers ab;
+  ab.dldi = ctx->a_dldi.as<float>();
+  ab.lrem = ctx->a_lrem.as<float4>();
+  ab.pend = ctx->a_pend.as<float4>();
+  ab.rec_key = ctx->a_key.as<uint32_t>();
+  ab.rec_g = ctx->a_g.as<float>();
+  ab.s_ga = ctx->a_sga.as<float4>();
+  ab.s_gb = ctx->a_sgb.as<float>();
+  ab.n_records = R;
+  ab.depth = D;
+  RayQueue q[2];
+  for (int i = 0; i < 2; ++i) {
+    q[i].o_tmin = ctx->q_o[i].as<float4>();
+    q[i].d_tmax = ctx->q_d[i].as<float4>();
+    q[i].t_pdf = ctx->q_t[i].as<float4>();
+    q[i].meta = ctx->q_m[i].as<uint32_t>();
+  }
+  HitBuf hb{ctx->h_t.as<float4>(), ctx->h_i.as<uint32_t>()};
+  ShadowQueue sq{ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>()};
+  hipStream_t stm = ctx->stream;
+  launch_zero_counters(ctr, stm);
+  launch_raygen_adj(rp, uint32_t(n), q[0], ctx->radiance.as<float4>(), ab, ctr, stm);
+  for (uint32_t d = 0; d < D; ++d) {
+    const int cur = int(d & 1u), nxt = cur ^ 1;
+    launch_extend(acc, q[cur], hb, &ctr[kCtrExt + d], &ctr[kCtrWorkExt + d], ctr, ctx->grid_trace, false, stm);
+    launch_shade_adj(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq, ab, &ctr[kCtrExt + d],
+                     &ctr[kCtrWorkShade + d], &ctr[kCtrExt + d + 1], &ctr[kCtrShadow + d], d, ctx->grid_shade, stm);
+    launch_shadow_adj(acc, sq, ab, &ctr[kCtrShadow + d], &ctr[kCtrWorkShadow + d], ctr, d, ctx->grid_trace, stm);
+  }
+  for (uint32_t lo = 0; lo < nm; lo += kGradMatsPerPass) {
+    const uint32_t cnt = std::min(kGradMatsPerPass, nm - lo);
+    launch_grad_reduce(ab, lo, cnt, ctx->a_partial.as<float>(), ctx->a_out.as<float>() + size_t(lo) * kNumParams, stm);
+  }
+  HIPC(hipGetLastError(), "adjoint launch");
+  HIPC(hipStreamSynchronize(stm), "adjoint");
+  uint32_t overflow = 0;
+  HIPC(hipMemcpy(&overflow, ctr + kCtrOverflow, 4, hipMemcpyDeviceToHost), "read overflow");
+  if (overflow) return fail(ctx, TRT_DEVICE, "traversal stack overflow");
+  HIPC(hipMemcpy(grad, ctx->a_out.p, size_t(nm) * kNumParams * 4, hipMemcpyDeviceToHost), "read gradient");
   return TRT_OK;
 }
 

@@ -66,6 +66,26 @@ struct FrameBuffers {
   float4* normal;     // per pixel first-hit shading normal (xyz), view depth (w)
 };
 
+// Adjoint (path replay) state. Records: one per (path, depth) slot r = path*depth + d,
+// key = material id of the vertex (0xffffffff = none), rec_g[k * n_records + r] =
+// gradient of parameter k (SoA so a wave's loads coalesce).
+struct AdjBuffers {
+  const float* dldi;   // dLoss/dImage, 3 floats per pixel
+  float4* lrem;        // per path: radiance not yet attributed to earlier vertices
+  float4* pend;        // per path: 4 float4 = (d f / d theta)/f of the pending continuation + material
+  uint32_t* rec_key;
+  float* rec_g;
+  float4* s_ga;        // per shadow slot: NEE gradient, params 0..3
+  float* s_gb;         // per shadow slot: NEE gradient, param 4
+  size_t n_records;
+  uint32_t depth;
+};
+constexpr int kGradBlocks = 1024;       // fixed: the reduction order is device independent
+constexpr uint32_t kGradMatsPerPass = 512;
+
+// Resident 256-thread blocks per CU of the traversal kernels for this accel's LDS stack.
+int trace_blocks_per_cu(const DevAccel& acc);
+
 // Kernel launchers (kernels.hip). All run on `stream`.
 void launch_zero_counters(uint32_t* ctr, hipStream_t stream);
 void launch_raygen(const RenderParams& rp, uint32_t n_paths, RayQueue q, float4* radiance, uint32_t* ctr,
@@ -79,6 +99,17 @@ void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, const 
                    uint32_t* ctr, int grid, bool stats, hipStream_t stream);
 void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
                      hipStream_t stream);
+void launch_raygen_adj(const RenderParams& rp, uint32_t n_paths, RayQueue q, const float4* radiance,
+                       const AdjBuffers& ab, uint32_t* ctr, hipStream_t stream);
+void launch_shade_adj(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
+                      HitBuf h, RayQueue qn, ShadowQueue sq, const AdjBuffers& ab, const uint32_t* count,
+                      uint32_t* work, uint32_t* next_count, uint32_t* shadow_count, uint32_t depth, int grid,
+                      hipStream_t stream);
+void launch_shadow_adj(const DevAccel& acc, ShadowQueue sq, const AdjBuffers& ab, const uint32_t* count,
+                       uint32_t* work, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream);
+// partial: kGradBlocks * n_mat * 5 floats; out: n_mat * 5 floats (device)
+void launch_grad_reduce(const AdjBuffers& ab, uint32_t mat_lo, uint32_t n_mat, float* partial, float* out,
+                        hipStream_t stream);
 // standalone traversal of caller rays (test hooks): rays as (o, tmin, d, tmax)
 void launch_trace(const DevAccel& acc, const float4* rays, uint32_t n, float4* hits, uint32_t* hit_inst,
                   uint32_t* occluded, bool any, uint32_t* ctr, int grid, hipStream_t stream);

@@ -433,9 +433,254 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevAccel acc, const float4* ra
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Adjoint (path-replay backpropagation). Gradient records live in fixed slots
+// r = path * D + depth, so their values never depend on queue order; the reduction
+// below sums them in slot order.
+__global__ __launch_bounds__(kBlock) void k_raygen_adj(RenderParams rp, uint32_t n_paths, RayQueue q,
+                                                        const float4* radiance, AdjBuffers ab, uint32_t* ctr) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  PathState ps;
+  bool alive = false;
+  if (p < n_paths) {
+    alive = raygen(rp, p, ps);
+    ab.lrem[p] = radiance[p];
+    for (uint32_t d = 0; d < ab.depth; ++d) ab.rec_key[static_cast<size_t>(p) * ab.depth + d] = kNoHit;
+    ab.pend[static_cast<size_t>(p) * 4 + 3] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kNoHit));
+  }
+  const uint64_t m = __ballot(alive);
+  const uint32_t base = wave_reserve(&ctr[kCtrExt + 0], static_cast<uint32_t>(__popcll(m)));
+  if (alive) {
+    const uint32_t s = base + prefix_count(m);
+    q.o_tmin[s] = make_float4(ps.o.x, ps.o.y, ps.o.z, rp.eps);
+    q.d_tmax[s] = make_float4(ps.d.x, ps.d.y, ps.d.z, 1e30f);
+    q.t_pdf[s] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.pdf);
+    q.meta[s] = (p << 2) | ps.flags;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade_adj(SceneView sv, const Material* sampling_mats, RenderParams rp,
+                                                       RayQueue q, HitBuf h, RayQueue qn, ShadowQueue sq,
+                                                       AdjBuffers ab, const uint32_t* count_p, uint32_t* work,
+                                                       uint32_t* next_count, uint32_t* shadow_count, uint32_t depth,
+                                                       uint32_t two_level) {
+  const uint32_t lane = lane_id();
+  const uint32_t count = *count_p;
+  const uint32_t npix = rp.width * rp.height;
+  const float inv_spp = 1.f / static_cast<float>(rp.spp);
+  const size_t R = ab.n_records;
+  for (;;) {
+    const uint32_t base = wave_fetch(work);
+    if (base >= count) break;
+    const uint32_t slot = base + lane;
+    ShadeOut out;
+    AdjOut adj;
+    out.has_ext = 0;
+    out.has_shadow = 0;
+    uint32_t path = 0;
+    if (slot < count) {
+      const float4 a = q.o_tmin[slot];
+      const float4 b = q.d_tmax[slot];
+      const float4 tp = q.t_pdf[slot];
+      const uint32_t meta = q.meta[slot];
+      const float4 hh = h.thit[slot];
+      PathState ps;
+      ps.o = v3(a.x, a.y, a.z);
+      ps.d = v3(b.x, b.y, b.z);
+      ps.T = v3(tp.x, tp.y, tp.z);
+      ps.pdf = tp.w;
+      ps.flags = meta & 3u;
+      ps.path = meta >> 2;
+      path = ps.path;
+      const uint32_t pix = path % npix;
+      const V3 delta = v3(ab.dldi[pix * 3 + 0] * inv_spp, ab.dldi[pix * 3 + 1] * inv_spp, ab.dldi[pix * 3 + 2] * inv_spp);
+      float4 lr = ab.lrem[path];
+      // continuation gradient of the previous vertex: the radiance still unaccounted
+      // for arrived through its sampled direction
+      if (depth > 0) {
+        const float4* pp = ab.pend + static_cast<size_t>(path) * 4;
+        const float4 p0 = pp[0], p1 = pp[1], p2 = pp[2], p3 = pp[3];
+        if (__float_as_uint(p3.w) != kNoHit) {
+          AdjOut prev;
+          prev.r[0][0] = p0.x; prev.r[0][1] = p0.y; prev.r[0][2] = p0.z;
+          prev.r[1][0] = p0.w; prev.r[1][1] = p1.x; prev.r[1][2] = p1.y;
+          prev.r[2][0] = p1.z; prev.r[2][1] = p1.w; prev.r[2][2] = p2.x;
+          prev.r[3][0] = p2.y; prev.r[3][1] = p2.z; prev.r[3][2] = p2.w;
+          prev.r[4][0] = p3.x; prev.r[4][1] = p3.y; prev.r[4][2] = p3.z;
+          float g[kNumParams];
+          continuation_grad(prev, delta, v3(lr.x, lr.y, lr.z), g);
+          const size_t r = static_cast<size_t>(path) * ab.depth + (depth - 1);
+          for (int k = 0; k < kNumParams; ++k) ab.rec_g[k * R + r] += g[k];
+        }
+      }
+      Hit hit;
+      hit.t = hh.x;
+      hit.b1 = hh.y;
+      hit.b2 = hh.z;
+      hit.prim = __float_as_uint(hh.w);
+      hit.inst = two_level ? h.inst[slot] : kNoInstance;
+      shade_adjoint(sv, sampling_mats, rp, ps, hit, depth, delta, out, adj);
+      if (!is_black(out.L)) {
+        lr.x -= out.L.x;
+        lr.y -= out.L.y;
+        lr.z -= out.L.z;
+        ab.lrem[path] = lr;
+      }
+      const size_t r = static_cast<size_t>(path) * ab.depth + depth;
+      if (adj.mat != kNoHit) {
+        ab.rec_key[r] = adj.mat;
+        for (int k = 0; k < kNumParams; ++k) ab.rec_g[k * R + r] = 0.f;
+      }
+      float4* pp = ab.pend + static_cast<size_t>(path) * 4;
+      if (out.has_ext) {
+        pp[0] = make_float4(adj.r[0][0], adj.r[0][1], adj.r[0][2], adj.r[1][0]);
+        pp[1] = make_float4(adj.r[1][1], adj.r[1][2], adj.r[2][0], adj.r[2][1]);
+        pp[2] = make_float4(adj.r[2][2], adj.r[3][0], adj.r[3][1], adj.r[3][2]);
+        pp[3] = make_float4(adj.r[4][0], adj.r[4][1], adj.r[4][2], __uint_as_float(adj.mat));
+      } else {
+        pp[3] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kNoHit));
+      }
+    }
+    const bool ext = out.has_ext != 0;
+    const bool shw = out.has_shadow != 0;
+    const uint64_t me = __ballot(ext);
+    const uint64_t ms = __ballot(shw);
+    const uint32_t be = wave_reserve(next_count, static_cast<uint32_t>(__popcll(me)));
+    const uint32_t bs = wave_reserve(shadow_count, static_cast<uint32_t>(__popcll(ms)));
+    if (ext) {
+      const uint32_t s = be + prefix_count(me);
+      qn.o_tmin[s] = make_float4(out.next.o.x, out.next.o.y, out.next.o.z, rp.eps);
+      qn.d_tmax[s] = make_float4(out.next.d.x, out.next.d.y, out.next.d.z, 1e30f);
+      qn.t_pdf[s] = make_float4(out.next.T.x, out.next.T.y, out.next.T.z, out.next.pdf);
+      qn.meta[s] = (path << 2) | out.next.flags;
+    }
+    if (shw) {
+      const uint32_t s = bs + prefix_count(ms);
+      sq.o_tmin[s] = make_float4(out.s_o.x, out.s_o.y, out.s_o.z, rp.eps);
+      sq.d_tmax[s] = make_float4(out.s_d.x, out.s_d.y, out.s_d.z, out.s_tmax);
+      sq.contrib[s] = make_float4(out.s_contrib.x, out.s_contrib.y, out.s_contrib.z, __uint_as_float(path));
+      ab.s_ga[s] = make_float4(adj.nee_g[0], adj.nee_g[1], adj.nee_g[2], adj.nee_g[3]);
+      ab.s_gb[s] = adj.nee_g[4];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shadow_adj(DevAccel acc, ShadowQueue sq, AdjBuffers ab,
+                                                        const uint32_t* count_p, uint32_t* work, uint32_t* ctr,
+                                                        uint32_t depth) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = lane_id();
+  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
+  const uint32_t count = *count_p;
+  const size_t R = ab.n_records;
+  for (;;) {
+    const uint32_t base = wave_fetch(work);
+    if (base >= count) break;
+    const uint32_t slot = base + lane;
+    if (slot < count) {
+      const float4 a = sq.o_tmin[slot];
+      const float4 b = sq.d_tmax[slot];
+      TraceOut r;
+      r.t = b.w;
+      r.prim = kNoHit;
+      r.inst = kNoInstance;
+      r.nodes = 0;
+      r.tests = 0;
+      const bool occ =
+          trace_ray<true, false>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
+      if (!occ) {
+        const float4 c = sq.contrib[slot];
+        const uint32_t path = __float_as_uint(c.w);
+        float4 lr = ab.lrem[path];
+        lr.x -= c.x;
+        lr.y -= c.y;
+        lr.z -= c.z;
+        ab.lrem[path] = lr;
+        const float4 ga = ab.s_ga[slot];
+        const float gb = ab.s_gb[slot];
+        const size_t rr = static_cast<size_t>(path) * ab.depth + depth;
+        ab.rec_g[0 * R + rr] += ga.x;
+        ab.rec_g[1 * R + rr] += ga.y;
+        ab.rec_g[2 * R + rr] += ga.z;
+        ab.rec_g[3 * R + rr] += ga.w;
+        ab.rec_g[4 * R + rr] += gb;
+      }
+    }
+  }
+}
+
+// Deterministic reduction of records into per-material gradients, materials
+// [mat_lo, mat_lo + n_mat). Block b owns records [b R/B, (b+1) R/B); each of its waves
+// a contiguous quarter; each wave adds its records in slot order into a private LDS
+// accumulator (one lane per parameter), then the block adds its four accumulators in
+// wave order. No float atomics anywhere: results are bitwise reproducible.
+__global__ __launch_bounds__(kBlock) void k_grad_reduce(const uint32_t* rec_key, const float* rec_g, uint64_t R,
+                                                         uint32_t mat_lo, uint32_t n_mat, float* partial) {
+  extern __shared__ float acc_lds[];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t P = n_mat * kNumParams;
+  float* acc = acc_lds + wave * P;
+  for (uint32_t i = lane; i < P; i += 64) acc[i] = 0.f;
+  const uint64_t b0 = R * blockIdx.x / gridDim.x, b1 = R * (blockIdx.x + 1) / gridDim.x;
+  const uint64_t w0 = b0 + (b1 - b0) * wave / kWavesPerBlock, w1 = b0 + (b1 - b0) * (wave + 1) / kWavesPerBlock;
+  for (uint64_t base = w0; base < w1; base += 64) {
+    const uint64_t r = base + lane;
+    uint32_t key = kNoHit;
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f;
+    if (r < w1) {
+      key = rec_key[r];
+      if (key >= mat_lo && key - mat_lo < n_mat) {
+        g0 = rec_g[0 * R + r];
+        g1 = rec_g[1 * R + r];
+        g2 = rec_g[2 * R + r];
+        g3 = rec_g[3 * R + r];
+        g4 = rec_g[4 * R + r];
+      } else {
+        key = kNoHit;
+      }
+    }
+    uint64_t mask = __ballot(key != kNoHit);
+    while (mask) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const uint32_t kj = __shfl(key, j);
+      const float v0 = __shfl(g0, j), v1 = __shfl(g1, j), v2 = __shfl(g2, j), v3v = __shfl(g3, j), v4 = __shfl(g4, j);
+      if (lane < kNumParams) {
+        const float v = lane == 0 ? v0 : lane == 1 ? v1 : lane == 2 ? v2 : lane == 3 ? v3v : v4;
+        acc[(kj - mat_lo) * kNumParams + lane] += v;
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < P; i += kBlock) {
+    float t = acc_lds[i];
+    for (int w = 1; w < kWavesPerBlock; ++w) t += acc_lds[w * P + i];
+    partial[static_cast<size_t>(blockIdx.x) * P + i] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grad_final(const float* partial, uint32_t n_blocks, uint32_t P,
+                                                        float* out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= P) return;
+  float t = 0.f;
+  for (uint32_t b = 0; b < n_blocks; ++b) t += partial[static_cast<size_t>(b) * P + i];
+  out[i] = t;
+}
+
 size_t stack_lds_bytes(const DevAccel& acc) { return static_cast<size_t>(acc.stack_cap) * 64u * 4u * kWavesPerBlock; }
 
 }  // namespace
+
+int trace_blocks_per_cu(const DevAccel& acc) {
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false>, kBlock, stack_lds_bytes(acc)) != hipSuccess)
+    a = 4;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kBlock, stack_lds_bytes(acc)) != hipSuccess)
+    b = 4;
+  const int m = a < b ? a : b;
+  return m > 0 ? m : 1;
+}
 
 void launch_zero_counters(uint32_t* ctr, hipStream_t stream) {
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(kNumCounters), 0, stream, ctr);
@@ -485,6 +730,36 @@ void launch_trace(const DevAccel& acc, const float4* rays, uint32_t n, float4* h
                   uint32_t* occluded, bool any, uint32_t* ctr, int grid, hipStream_t stream) {
   hipLaunchKernelGGL(k_trace, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, rays, n, hits, hit_inst,
                      occluded, any ? 1u : 0u, ctr);
+}
+
+
+void launch_raygen_adj(const RenderParams& rp, uint32_t n_paths, RayQueue q, const float4* radiance,
+                       const AdjBuffers& ab, uint32_t* ctr, hipStream_t stream) {
+  const uint32_t blocks = (n_paths + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_raygen_adj, dim3(blocks), dim3(kBlock), 0, stream, rp, n_paths, q, radiance, ab, ctr);
+}
+
+void launch_shade_adj(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
+                      HitBuf h, RayQueue qn, ShadowQueue sq, const AdjBuffers& ab, const uint32_t* count,
+                      uint32_t* work, uint32_t* next_count, uint32_t* shadow_count, uint32_t depth, int grid,
+                      hipStream_t stream) {
+  hipLaunchKernelGGL(k_shade_adj, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, ab, count,
+                     work, next_count, shadow_count, depth, sv.instances ? 1u : 0u);
+}
+
+void launch_shadow_adj(const DevAccel& acc, ShadowQueue sq, const AdjBuffers& ab, const uint32_t* count,
+                       uint32_t* work, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_shadow_adj, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, ab, count, work,
+                     ctr, depth);
+}
+
+void launch_grad_reduce(const AdjBuffers& ab, uint32_t mat_lo, uint32_t n_mat, float* partial, float* out,
+                        hipStream_t stream) {
+  const uint32_t P = n_mat * kNumParams;
+  hipLaunchKernelGGL(k_grad_reduce, dim3(kGradBlocks), dim3(kBlock), P * 4 * kWavesPerBlock, stream, ab.rec_key,
+                     ab.rec_g, static_cast<uint64_t>(ab.n_records), mat_lo, n_mat, partial);
+  hipLaunchKernelGGL(k_grad_final, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, partial,
+                     static_cast<uint32_t>(kGradBlocks), P, out);
 }
 
 }  // namespace trt

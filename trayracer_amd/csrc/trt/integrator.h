@@ -94,17 +94,29 @@ TRT_FI float light_pdf_hit(const SceneView& s, V3 le, float area, float cos_l, f
   return pick * (t * t) / (area * cos_l);
 }
 
-// One vertex: ps = incoming path state, h = closest hit of ray (ps.o, ps.d).
-// sampling_mats may differ from s.materials only in differentiable renders
-// (sampling decisions use it, BSDF values use s.materials).
-TRT_HD inline void shade(const SceneView& s, const Material* sampling_mats, const RenderParams& rp,
-                         const PathState& ps, const Hit& h, uint32_t depth, ShadeOut& out) {
+// Adjoint extras of one vertex (path-replay backpropagation, Vicini et al. 2021):
+// derivatives of the vertex's terms with respect to its material's differentiable
+// parameters theta = (baseColor r, g, b, roughness, metallic). Sampling decisions and
+// MIS weights use the sampling materials and are treated as constants ("detached"),
+// so the derivative of a BSDF-sampled weight f cos / pdf is (df/dtheta) cos / pdf.
+constexpr int kNumParams = 5;
+struct AdjOut {
+  uint32_t mat;                 // material of the vertex (kNoHit when none)
+  float nee_g[kNumParams];      // sum_c delta_c d(NEE contribution_c)/d theta_k
+  float r[kNumParams][3];       // (d f_c / d theta_k) / f_c at the sampled extension
+};
+
+template <bool ADJ>
+TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats, const RenderParams& rp,
+                              const PathState& ps, const Hit& h, uint32_t depth, ShadeOut& out, V3 delta,
+                              AdjOut* adj) {
   out.L = v3(0.f);
   out.has_shadow = 0;
   out.has_ext = 0;
   out.albedo = v3(0.f);
   out.normal = v3(0.f);
   out.vdepth = 0.f;
+  if (ADJ) adj->mat = kNoHit;
   if (h.prim == kNoHit) return;  // no sky: the configured scenes are emitter-lit
 
   const SurfacePoint sp = surface_at(s, h, ps.o, ps.d);
@@ -142,6 +154,24 @@ TRT_HD inline void shade(const SceneView& s, const Material* sampling_mats, cons
   if (!(wo.z > 0.f)) return;
   const BsdfParams<float> bp = params_of(m);
   const BsdfParams<float> bs = params_of(ms);
+  BsdfParams<Dual<kNumParams>> bd;
+  if (ADJ) {
+    adj->mat = sp.mat;
+    for (int k = 0; k < kNumParams; ++k) adj->nee_g[k] = 0.f;
+    for (int k = 0; k < kNumParams; ++k) adj->r[k][0] = adj->r[k][1] = adj->r[k][2] = 0.f;
+    bd.base = t3(dvar<kNumParams>(m.base[0], 0), dvar<kNumParams>(m.base[1], 1), dvar<kNumParams>(m.base[2], 2));
+    bd.roughness = dvar<kNumParams>(m.roughness, 3);
+    bd.metallic = dvar<kNumParams>(m.metallic, 4);
+    bd.specular = bp.specular;
+    bd.specular_tint = bp.specular_tint;
+    bd.sheen = bp.sheen;
+    bd.sheen_tint = bp.sheen_tint;
+    bd.clearcoat = bp.clearcoat;
+    bd.clearcoat_gloss = bp.clearcoat_gloss;
+    bd.subsurface = bp.subsurface;
+    bd.anisotropic = bp.anisotropic;
+    bd.model = bp.model;
+  }
 
   // next-event estimation
   if (s.n_lights > 0 && s.total_power > 0.f) {
@@ -175,6 +205,13 @@ TRT_HD inline void shade(const SceneView& s, const Material* sampling_mats, cons
             out.s_d = wl;
             out.s_tmax = dist - 2.f * rp.eps;
             out.s_contrib = clamp_contrib(c, rp.clamp);
+            if (ADJ) {
+              const T3<Dual<kNumParams>> fd = bsdf_f(bd, wo, wi);
+              const float kx = delta.x * ps.T.x * L.le[0] * k, ky = delta.y * ps.T.y * L.le[1] * k,
+                          kz = delta.z * ps.T.z * L.le[2] * k;
+              for (int j = 0; j < kNumParams; ++j)
+                adj->nee_g[j] = (kx * fd.x.d[j] + ky * fd.y.d[j]) + kz * fd.z.d[j];
+            }
           }
         }
       }
@@ -204,10 +241,40 @@ TRT_HD inline void shade(const SceneView& s, const Material* sampling_mats, cons
           out.next.pdf = smp.pdf;
           out.next.flags = 0u;
           out.next.path = ps.path;
+          if (ADJ) {
+            const T3<Dual<kNumParams>> fd = bsdf_f(bd, wo, smp.wi);
+            for (int j = 0; j < kNumParams; ++j) {
+              adj->r[j][0] = f.x != 0.f ? fd.x.d[j] / f.x : 0.f;
+              adj->r[j][1] = f.y != 0.f ? fd.y.d[j] / f.y : 0.f;
+              adj->r[j][2] = f.z != 0.f ? fd.z.d[j] / f.z : 0.f;
+            }
+          }
         }
       }
     }
   }
+}
+
+// One vertex: ps = incoming path state, h = closest hit of ray (ps.o, ps.d).
+// sampling_mats may differ from s.materials only in differentiable renders
+// (sampling decisions use it, BSDF values use s.materials).
+TRT_HD inline void shade(const SceneView& s, const Material* sampling_mats, const RenderParams& rp,
+                         const PathState& ps, const Hit& h, uint32_t depth, ShadeOut& out) {
+  shade_impl<false>(s, sampling_mats, rp, ps, h, depth, out, v3(0.f), nullptr);
+}
+
+// Same vertex replayed for the adjoint: delta = dLoss/d(path radiance).
+TRT_HD inline void shade_adjoint(const SceneView& s, const Material* sampling_mats, const RenderParams& rp,
+                                 const PathState& ps, const Hit& h, uint32_t depth, V3 delta, ShadeOut& out,
+                                 AdjOut& adj) {
+  shade_impl<true>(s, sampling_mats, rp, ps, h, depth, out, delta, &adj);
+}
+
+// Gradient of the BSDF-sampled continuation of a vertex once the radiance that
+// arrived through it (l_after) is known: sum_c delta_c r_kc l_after_c.
+TRT_FI void continuation_grad(const AdjOut& a, V3 delta, V3 l_after, float* g) {
+  const float x = delta.x * l_after.x, y = delta.y * l_after.y, z = delta.z * l_after.z;
+  for (int k = 0; k < kNumParams; ++k) g[k] = (a.r[k][0] * x + a.r[k][1] * y) + a.r[k][2] * z;
 }
 
 }  // namespace trt

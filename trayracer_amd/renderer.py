@@ -89,6 +89,7 @@ class Renderer:
         lib, ctx = self.lib, self.ctx
         self._check(lib.trt_clear_scene(ctx))
         mats = np.ascontiguousarray(scene["materials"], np.float32)
+        self.n_materials = len(mats)
         cm = (_lib.TrtMaterial * len(mats))()
         ctypes.memmove(cm, mats.ctypes.data, mats.nbytes)
         self._check(lib.trt_set_materials(ctx, cm, len(mats)))
@@ -178,6 +179,33 @@ class Renderer:
         out = np.empty((n, 3), np.float32)
         self._check(self.lib.trt_read_path_radiance(self.ctx, _fp(out), n))
         return out
+
+    # -- differentiable rendering -----------------------------------------------------
+    EVAL, SAMPLING, BOTH = 1, 2, 3
+
+    def set_material_params(self, params, which=3):
+        """params [n_materials, 5]: base colour rgb, roughness, metallic."""
+        params = np.ascontiguousarray(params, np.float32).reshape(-1, 5)
+        self._check(self.lib.trt_set_material_params(self.ctx, _fp(params), params.shape[0], which))
+
+    def backward(self, dl_dimage):
+        """Gradient of a loss w.r.t. every material's 5 parameters for the last frame;
+        dl_dimage = dLoss/d(frame mean image) [H, W, 3]."""
+        s = self.settings
+        g = np.ascontiguousarray(dl_dimage, np.float32).reshape(s.height, s.width, 3)
+        out = np.empty((self.n_materials, 5), np.float32)
+        o = s.to_c()
+        self._check(self.lib.trt_backward(self.ctx, ctypes.byref(o), _fp(g), _fp(out)))
+        return out
+
+    def frame_image(self):
+        """Mean of the last frame's paths per pixel (= color() after a single frame)."""
+        s = self.settings
+        L = self.path_radiance().reshape(s.spp, s.height, s.width, 3).astype(np.float32)
+        acc = np.zeros((s.height, s.width, 3), np.float32)
+        for k in range(s.spp):
+            acc += L[k]
+        return acc * np.float32(1.0 / s.spp)
 
     # -- ray queries -----------------------------------------------------------------
     def trace_closest(self, rays):

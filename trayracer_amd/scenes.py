@@ -105,6 +105,38 @@ def cornell(width=256, height=256, spp=16, max_depth=4, rr_depth=3, seed=0):
     )
 
 
+PARAM_LO = np.array([0.0, 0.0, 0.0, 0.05, 0.0], np.float32)
+PARAM_HI = np.array([1.0, 1.0, 1.0, 1.0, 1.0], np.float32)
+
+
+def material_params(materials):
+    """[M, 5] differentiable parameters (base colour rgb, roughness, metallic)."""
+    m = np.asarray(materials, np.float32)
+    return np.concatenate([m[:, 0:3], m[:, 3:4], m[:, 7:8]], axis=1).copy()
+
+
+def cornell_diff(width=512, height=512, spp=64, max_depth=4, seed=3, sigma=0.1):
+    """Differentiable Cornell box: the Cornell geometry with one Disney material per
+    triangle (36 x 5 = 180 parameters: base colour rgb, roughness, metallic). Returns
+    (scene, target_params): the target is the scene's parameters perturbed by
+    N(0, sigma) and clipped to their valid ranges. Russian roulette starts at the last
+    vertex (rr_depth = max_depth - 1), where no extension is traced, so the estimator is
+    a smooth function of the parameters."""
+    sc = cornell(width=width, height=height, spp=spp, max_depth=max_depth, rr_depth=max_depth - 1, seed=seed)
+    base = sc["materials"][sc["mat_ids"]].copy()
+    rng = np.random.default_rng(seed)
+    for i, m in enumerate(base):
+        m[16:17] = np.array([MODEL_DISNEY], np.uint32).view(np.float32)
+        if m[4:7].sum() == 0:  # non-emitters get varied roughness / metallic
+            m[3] = rng.uniform(0.3, 0.9)
+            m[7] = 1.0 if m[7] > 0.5 else rng.uniform(0.0, 0.3)
+    sc["materials"] = base
+    sc["mat_ids"] = np.arange(36, dtype=np.uint32)
+    p = material_params(base)
+    target = np.clip(p + rng.normal(0.0, sigma, p.shape).astype(np.float32), PARAM_LO, PARAM_HI)
+    return sc, target
+
+
 def _unit_vectors(rng, n):
     v = rng.normal(size=(n, 3))
     return v / np.linalg.norm(v, axis=1, keepdims=True)
