@@ -211,6 +211,43 @@ trt_status trt_set_material_params(trt_context* ctx, const float* params, uint32
  * in a fixed order (no float atomics): repeated calls return identical bits. */
 trt_status trt_backward(trt_context* ctx, const trt_render_opts* opts, const float* dL_dimage, float* dL_dparams);
 
+/* Learned denoiser (replaces Denoiser, Tracer/Optix/Denoiser.h:14-38 and its
+ * DenoiseFrame, Tracer/Optix/Denoiser.cpp:47-115, which run the closed OptiX AI
+ * denoiser on colour + albedo + normal). A 4-level U-Net (32/64/96/128 channels, 3x3
+ * convolutions, BF16 weights/activations, FP32 accumulation on MFMA) over 10 input
+ * channels: log(1 + radiance), first-hit albedo, normal, log(1 + view depth). */
+typedef struct trt_denoise_stats {
+  float features_ms;
+  float network_ms;
+  float total_ms;
+} trt_denoise_stats;
+trt_status trt_denoiser_init(trt_context* ctx, uint32_t seed);
+uint64_t trt_denoiser_num_params(void);
+trt_status trt_denoiser_get_params(trt_context* ctx, float* out, uint64_t n);
+trt_status trt_denoiser_set_params(trt_context* ctx, const float* in, uint64_t n);
+trt_status trt_denoiser_save(trt_context* ctx, const char* path);
+trt_status trt_denoiser_load(trt_context* ctx, const char* path);
+/* Training set of n (input, target) pairs of width x height, captured from frames the
+ * context renders: kind 0 stores the current frame's input features, kind 1 its mean
+ * colour as the target (render the target at high spp). */
+trt_status trt_denoiser_dataset(trt_context* ctx, uint32_t n, uint32_t width, uint32_t height);
+trt_status trt_denoiser_capture(trt_context* ctx, uint32_t slot, uint32_t kind);
+/* `steps` Adam steps on random crop x crop windows (crop % 128 == 0), batch crops per
+ * step; losses (nullable) receives one mean squared log-space error per step. */
+trt_status trt_denoiser_train(trt_context* ctx, uint32_t steps, uint32_t batch, uint32_t crop, float lr, uint32_t seed,
+                              float* losses);
+/* Denoises the accumulated frame (needs width % 128 == 0, height % 8 == 0). */
+trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* stats);
+trt_status trt_read_denoised(trt_context* ctx, float* rgba);
+/* Test hooks: network forward on host features [N][H][W][16] -> log-space output
+ * [N][H][W][32]; backward of sum(dout * output) -> parameter gradients. */
+trt_status trt_denoiser_forward(trt_context* ctx, const float* features, uint32_t n, uint32_t height, uint32_t width,
+                                float* out);
+trt_status trt_denoiser_backward(trt_context* ctx, const float* dout, uint32_t n, uint32_t height, uint32_t width,
+                                 float* grads);
+/* Post-activation output of convolution `index` (0..13) of the last forward. */
+trt_status trt_denoiser_activation(trt_context* ctx, uint32_t index, float* out, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,138 @@
+"""GPU U-Net denoiser (BF16 MFMA convolutions) against the FP64 reference network of
+the oracle, which rounds to bfloat16 at the same points: forward output and every
+parameter gradient. Then training and denoising end to end on rendered frames."""
+import numpy as np
+import pytest
+
+from trayracer_amd import Renderer, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rend():
+    r = Renderer(0)
+    yield r
+    r.close()
+
+
+def _rand_features(rng, N, H, W):
+    f = np.zeros((N, H, W, 16), np.float32)
+    f[..., :10] = rng.normal(0.5, 0.5, (N, H, W, 10))
+    return f
+
+
+def test_forward_and_backward_match_reference(rend):
+    from oracle import unet_ref as U
+    rng = np.random.default_rng(0)
+    params = U.init_params(7)
+    rend.denoiser_set_params(params)
+    N, H, W = 2, 16, 128
+    feats = _rand_features(rng, N, H, W)
+    out = rend.denoiser_forward(feats)
+    ref = U.UnetRef(params, round_bf16=True)
+    want = ref.forward(feats)
+    err = np.abs(out - want).max() / np.abs(want).max()
+    print("forward max error / max |ref|:", err)
+    assert err < 2e-2
+    assert np.sqrt(((out - want) ** 2).mean()) / np.sqrt((want ** 2).mean()) < 5e-3
+    # each layer against the reference applied to the device's own input activation
+    shapes = {}
+    for i, (ci, co, _) in enumerate(U.SPEC[:14]):
+        lvl = {0: 1, 1: 1, 2: 2, 3: 2, 4: 4, 5: 4, 6: 8, 7: 8, 8: 4, 9: 4, 10: 2, 11: 2, 12: 1, 13: 1}[i]
+        shapes[i] = (N, H // lvl, W // lvl, co)
+    acts = {i: rend.denoiser_activation(i, shapes[i]) for i in range(14)}
+    layer_err = 0.0
+    for i in range(1, 14):
+        prev = {1: acts[0], 2: U.pool(acts[1], U.bf16), 3: acts[2], 4: U.pool(acts[3], U.bf16), 5: acts[4],
+                6: U.pool(acts[5], U.bf16), 7: acts[6], 8: U.upcat(acts[7], acts[5]), 9: acts[8],
+                10: U.upcat(acts[9], acts[3]), 11: acts[10], 12: U.upcat(acts[11], acts[1]), 13: acts[12]}[i]
+        w, b = ref.layers[i]
+        y = U.bf16(np.maximum(U.conv3x3(prev, w, b), 0.0))
+        d = np.abs(y - acts[i]) / np.maximum(np.abs(y).max(), 1e-30)
+        layer_err = max(layer_err, float(d.max()))
+    print("per-layer forward max error:", layer_err)
+    assert layer_err < 1e-2  # one bf16 ulp of the largest activation
+    dout = rng.normal(size=(N, H, W, 32)).astype(np.float32)
+    dout[..., 3:] = 0
+    g = rend.denoiser_backward(dout)
+    ref.use_activations(feats, acts)
+    gr = ref.backward(dout.astype(np.float64))
+    offs = np.cumsum([0] + [co * 9 * ci + co for ci, co, _ in U.SPEC])
+    worst = 0.0
+    for li, (ci, co, _) in enumerate(U.SPEC):
+        a, b = g[offs[li]:offs[li + 1]], gr[offs[li]:offs[li + 1]]
+        nw = co * 9 * ci
+        rw = np.linalg.norm(a[:nw] - b[:nw]) / max(np.linalg.norm(b[:nw]), 1e-30)
+        rb = np.linalg.norm(a[nw:] - b[nw:]) / max(np.linalg.norm(b[nw:]), 1e-30)
+        wa, wb = a[:nw].reshape(co, 9, ci), b[:nw].reshape(co, 9, ci)
+        per_tap = [np.linalg.norm(wa[:, t] - wb[:, t]) / max(np.linalg.norm(wb[:, t]), 1e-30) for t in range(9)]
+        print(f"layer {li}: weight grad error {rw:.2e}, bias grad error {rb:.2e}, per tap "
+              + " ".join(f"{x:.1e}" for x in per_tap))
+        worst = max(worst, rw, rb)
+    assert worst < 1e-2, worst
+
+
+def test_train_and_denoise_cornell(rend, tmp_path):
+    W = H = 256
+    rend.denoiser_init(seed=1)
+    n_pairs = 4
+    rend.denoiser_dataset(n_pairs, W, H)
+    for k in range(n_pairs):
+        sc = scenes.cornell(width=W, height=H, spp=1, max_depth=4, seed=100 + k)
+        rend.UpdateScene(sc)
+        rend.RenderFrame()
+        rend.denoiser_capture(k, 0)
+        rend.settings.spp = 64
+        rend.settings.seed = 1000 + k
+        rend.Reset()
+        for _ in range(4):
+            rend.RenderFrame()
+        rend.denoiser_capture(k, 1)
+    losses = rend.denoiser_train(120, batch=4, crop=128, lr=2e-3, seed=3)
+    print("loss first/last:", losses[:3], losses[-3:])
+    assert np.isfinite(losses).all()
+    assert losses[-10:].mean() < 0.5 * losses[:5].mean()
+    # held-out frame: noisy 1 spp vs denoised, both against a 512-spp render
+    sc = scenes.cornell(width=W, height=H, spp=64, max_depth=4, seed=7)
+    rend.UpdateScene(sc)
+    for _ in range(8):
+        rend.RenderFrame()
+    ref = rend.color()[..., :3].astype(np.float64)
+    rend.settings.spp = 1
+    rend.settings.seed = 77
+    rend.Reset()
+    rend.RenderFrame()
+    noisy = rend.color()[..., :3].astype(np.float64)
+    st = rend.denoise()
+    den = rend.denoised()[..., :3].astype(np.float64)
+
+    def relmse(x):
+        return float(((x - ref) ** 2 / (ref ** 2 + 1e-2)).mean())
+    print("relMSE noisy", relmse(noisy), "denoised", relmse(den), "denoise ms", st)
+    assert relmse(den) < 0.5 * relmse(noisy)
+    # weights round-trip through the own file format
+    w = rend.denoiser_params()
+    path = tmp_path / "unet.trtdn"
+    rend.denoiser_save(path)
+    rend.denoiser_init(seed=9)
+    rend.denoiser_load(path)
+    assert np.array_equal(rend.denoiser_params(), w)
+
+
+def test_training_is_deterministic(rend):
+    W = H = 128
+    rend.denoiser_init(seed=5)
+    rend.denoiser_dataset(1, W, H)
+    sc = scenes.cornell(width=W, height=H, spp=1, max_depth=3, seed=1)
+    rend.UpdateScene(sc)
+    rend.RenderFrame()
+    rend.denoiser_capture(0, 0)
+    rend.RenderFrame()
+    rend.denoiser_capture(0, 1)
+    l1 = rend.denoiser_train(5, batch=2, crop=128, lr=1e-3, seed=11)
+    w1 = rend.denoiser_params()
+    rend.denoiser_init(seed=5)
+    l2 = rend.denoiser_train(5, batch=2, crop=128, lr=1e-3, seed=11)
+    assert np.array_equal(l1, l2)
+    assert np.array_equal(rend.denoiser_params(), w1)

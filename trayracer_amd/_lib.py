@@ -79,6 +79,13 @@ class TrtPickResult(ctypes.Structure):
                 ("instance", ctypes.c_uint32), ("primitive", ctypes.c_uint32)]
 
 
+class TrtDenoiseStats(ctypes.Structure):
+    _fields_ = [("features_ms", ctypes.c_float), ("network_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class TrtTraceCounts(ctypes.Structure):
     _fields_ = [
         ("closest_nodes_per_ray", ctypes.c_double), ("closest_tris_per_ray", ctypes.c_double),
@@ -121,6 +128,21 @@ PROTOTYPES = [
     ("trt_measure_traversal", S, [P, ctypes.POINTER(TrtRenderOpts), ctypes.POINTER(TrtTraceCounts)]),
     ("trt_set_material_params", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("trt_backward", S, [P, ctypes.POINTER(TrtRenderOpts), c_float_p, c_float_p]),
+    ("trt_denoiser_init", S, [P, ctypes.c_uint32]),
+    ("trt_denoiser_num_params", ctypes.c_uint64, []),
+    ("trt_denoiser_get_params", S, [P, c_float_p, ctypes.c_uint64]),
+    ("trt_denoiser_set_params", S, [P, c_float_p, ctypes.c_uint64]),
+    ("trt_denoiser_save", S, [P, ctypes.c_char_p]),
+    ("trt_denoiser_load", S, [P, ctypes.c_char_p]),
+    ("trt_denoiser_dataset", S, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    ("trt_denoiser_capture", S, [P, ctypes.c_uint32, ctypes.c_uint32]),
+    ("trt_denoiser_train", S, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32,
+                               c_float_p]),
+    ("trt_denoise", S, [P, ctypes.POINTER(TrtDenoiseStats)]),
+    ("trt_read_denoised", S, [P, c_float_p]),
+    ("trt_denoiser_forward", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
+    ("trt_denoiser_activation", S, [P, ctypes.c_uint32, c_float_p, ctypes.c_uint64]),
+    ("trt_denoiser_backward", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
 ]
 
 STATUS = {0: "TRT_OK", 1: "TRT_INVALID_ARG", 2: "TRT_OUT_OF_MEMORY", 3: "TRT_DEVICE", 4: "TRT_NOT_BUILT",

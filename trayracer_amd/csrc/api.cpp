@@ -10,6 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <memory>
+#include <random>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -18,6 +21,7 @@
 
 #include "../../include/trt.h"
 #include "bvh_build.h"
+#include "denoiser.h"
 #include "device.h"
 #include "trt/lights_build.h"
 
@@ -114,6 +118,12 @@ struct trt_context {
   trt_render_opts last_opts{};
   uint32_t last_sample_offset = 0;
   DevBuf a_lrem, a_pend, a_key, a_g, a_sga, a_sgb, a_dldi, a_partial, a_out;
+
+  // denoiser
+  std::unique_ptr<Unet> unet;
+  DevBuf dn_feat, dn_denoised, dn_ds_feat, dn_ds_tgt, dn_batch_feat, dn_batch_tgt, dn_crops, dn_io;
+  uint32_t dn_ds_n = 0, dn_ds_w = 0, dn_ds_h = 0;
+  std::vector<uint8_t> dn_ds_have;  // per slot: bit0 input, bit1 target
 };
 
 namespace {
@@ -911,6 +921,240 @@ trt_status trt_backward(trt_context* ctx, const trt_render_opts* o, const float*
   HIPC(hipMemcpy(&overflow, ctr + kCtrOverflow, 4, hipMemcpyDeviceToHost), "read overflow");
   if (overflow) return fail(ctx, TRT_DEVICE, "traversal stack overflow");
   HIPC(hipMemcpy(grad, ctx->a_out.p, size_t(nm) * kNumParams * 4, hipMemcpyDeviceToHost), "read gradient");
+  return TRT_OK;
+}
+
+
+// ---------------------------------------------------------------------------------
+// Denoiser
+// ---------------------------------------------------------------------------------
+static uint16_t host_f2bf(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+static trt_status need_unet(trt_context* ctx) {
+  if (!ctx) return TRT_INVALID_ARG;
+  if (!ctx->unet) return fail(ctx, TRT_NOT_BUILT, "denoiser not initialised (trt_denoiser_init)");
+  return TRT_OK;
+}
+
+trt_status trt_denoiser_init(trt_context* ctx, uint32_t seed) {
+  if (!ctx) return TRT_INVALID_ARG;
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  if (!ctx->unet) ctx->unet.reset(new Unet());
+  HIPC(ctx->unet->init(seed, ctx->stream), "denoiser init");
+  return TRT_OK;
+}
+
+uint64_t trt_denoiser_num_params(void) {
+  static const uint64_t n = Unet().num_params();
+  return n;
+}
+
+trt_status trt_denoiser_get_params(trt_context* ctx, float* out, uint64_t n) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (!out || n != ctx->unet->num_params()) return fail(ctx, TRT_INVALID_ARG, "n must equal trt_denoiser_num_params()");
+  HIPC(ctx->unet->get_params(out, ctx->stream), "denoiser params");
+  return TRT_OK;
+}
+
+trt_status trt_denoiser_set_params(trt_context* ctx, const float* in, uint64_t n) {
+  if (!ctx || !in) return TRT_INVALID_ARG;
+  if (!ctx->unet) {
+    trt_status s = trt_denoiser_init(ctx, 0);
+    if (s != TRT_OK) return s;
+  }
+  if (n != ctx->unet->num_params()) return fail(ctx, TRT_INVALID_ARG, "n must equal trt_denoiser_num_params()");
+  if (!finite_all(in, n)) return fail(ctx, TRT_INVALID_ARG, "non-finite denoiser parameter");
+  HIPC(ctx->unet->set_params(in, ctx->stream), "denoiser params");
+  return TRT_OK;
+}
+
+// Own versioned format: magic "TRTDN001", uint64 count, count float32 values.
+trt_status trt_denoiser_save(trt_context* ctx, const char* path) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (!path) return TRT_INVALID_ARG;
+  std::vector<float> w(ctx->unet->num_params());
+  HIPC(ctx->unet->get_params(w.data(), ctx->stream), "denoiser params");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(ctx, TRT_INVALID_ARG, std::string("cannot open ") + path);
+  const uint64_t n = w.size();
+  bool ok = std::fwrite("TRTDN001", 1, 8, f) == 8 && std::fwrite(&n, 8, 1, f) == 1 &&
+            std::fwrite(w.data(), 4, w.size(), f) == w.size();
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? TRT_OK : fail(ctx, TRT_DEVICE, "write failed");
+}
+
+trt_status trt_denoiser_load(trt_context* ctx, const char* path) {
+  if (!ctx || !path) return TRT_INVALID_ARG;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(ctx, TRT_INVALID_ARG, std::string("cannot open ") + path);
+  char magic[8];
+  uint64_t n = 0;
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "TRTDN001", 8) == 0 && std::fread(&n, 8, 1, f) == 1 &&
+            n == trt_denoiser_num_params();
+  std::vector<float> w(ok ? n : 0);
+  ok = ok && std::fread(w.data(), 4, n, f) == n;
+  std::fclose(f);
+  if (!ok) return fail(ctx, TRT_INVALID_ARG, "not a denoiser weight file of this network");
+  return trt_denoiser_set_params(ctx, w.data(), n);
+}
+
+trt_status trt_denoiser_dataset(trt_context* ctx, uint32_t n, uint32_t width, uint32_t height) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (n == 0 || width == 0 || height == 0) return fail(ctx, TRT_INVALID_ARG, "empty dataset");
+  const uint64_t P = uint64_t(n) * width * height;
+  HIPC(ctx->dn_ds_feat.ensure(P * kFeatures * 2), "alloc dataset");
+  HIPC(ctx->dn_ds_tgt.ensure(P * 3 * 4), "alloc dataset");
+  ctx->dn_ds_n = n;
+  ctx->dn_ds_w = width;
+  ctx->dn_ds_h = height;
+  ctx->dn_ds_have.assign(n, 0);
+  return TRT_OK;
+}
+
+trt_status trt_denoiser_capture(trt_context* ctx, uint32_t slot, uint32_t kind) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (slot >= ctx->dn_ds_n || kind > 1) return fail(ctx, TRT_INVALID_ARG, "slot or kind out of range");
+  if (ctx->width != ctx->dn_ds_w || ctx->height != ctx->dn_ds_h || ctx->sample_count == 0)
+    return fail(ctx, TRT_INVALID_ARG, "render a frame of the dataset's size first");
+  const uint64_t P = uint64_t(ctx->width) * ctx->height;
+  if (kind == 0)
+    dn_features(ctx->color.as<float4>(), ctx->albedo.as<float4>(), ctx->normal.as<float4>(),
+                ctx->dn_ds_feat.as<uint16_t>() + size_t(slot) * P * kFeatures, P, ctx->stream);
+  else
+    dn_target(ctx->color.as<float4>(), ctx->dn_ds_tgt.as<float>() + size_t(slot) * P * 3, P, ctx->stream);
+  HIPC(hipGetLastError(), "capture");
+  HIPC(hipStreamSynchronize(ctx->stream), "capture");
+  ctx->dn_ds_have[slot] |= uint8_t(1u << kind);
+  return TRT_OK;
+}
+
+trt_status trt_denoiser_train(trt_context* ctx, uint32_t steps, uint32_t batch, uint32_t crop, float lr, uint32_t seed,
+                              float* losses) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (ctx->dn_ds_n == 0) return fail(ctx, TRT_INVALID_ARG, "no dataset");
+  for (uint8_t h : ctx->dn_ds_have)
+    if (h != 3) return fail(ctx, TRT_INVALID_ARG, "every dataset slot needs an input and a target");
+  if (batch == 0 || crop % 128 || crop > ctx->dn_ds_w || crop > ctx->dn_ds_h || !(lr > 0.f))
+    return fail(ctx, TRT_INVALID_ARG, "crop must be a multiple of 128 within the dataset images; batch, lr > 0");
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  const uint64_t P = uint64_t(batch) * crop * crop;
+  HIPC(ctx->dn_batch_feat.ensure(P * kFeatures * 2), "alloc batch");
+  HIPC(ctx->dn_batch_tgt.ensure(P * 3 * 4), "alloc batch");
+  HIPC(ctx->dn_crops.ensure(size_t(batch) * 16), "alloc batch");
+  std::mt19937 rng(seed);
+  std::vector<int> crops(size_t(batch) * 4);
+  for (uint32_t it = 0; it < steps; ++it) {
+    for (uint32_t b = 0; b < batch; ++b) {
+      crops[b * 4 + 0] = int(rng() % ctx->dn_ds_n);
+      crops[b * 4 + 1] = int(rng() % (ctx->dn_ds_w - crop + 1));
+      crops[b * 4 + 2] = int(rng() % (ctx->dn_ds_h - crop + 1));
+      crops[b * 4 + 3] = 0;
+    }
+    HIPC(hipMemcpyAsync(ctx->dn_crops.p, crops.data(), crops.size() * 4, hipMemcpyHostToDevice, ctx->stream),
+         "upload crops");
+    dn_gather_crop(ctx->dn_ds_feat.as<uint16_t>(), ctx->dn_ds_tgt.as<float>(), int(ctx->dn_ds_h), int(ctx->dn_ds_w),
+                   ctx->dn_crops.as<int4>(), int(batch), int(crop), ctx->dn_batch_feat.as<uint16_t>(),
+                   ctx->dn_batch_tgt.as<float>(), ctx->stream);
+    float loss = 0.f;
+    HIPC(ctx->unet->train_step(ctx->dn_batch_feat.as<uint16_t>(), ctx->dn_batch_tgt.as<float>(), int(batch), int(crop),
+                               int(crop), lr, ctx->stream, &loss),
+         "train step");
+    if (losses) losses[it] = loss;
+  }
+  return TRT_OK;
+}
+
+trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* st) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (ctx->width == 0 || ctx->sample_count == 0) return fail(ctx, TRT_INVALID_ARG, "nothing rendered yet");
+  if (!ctx->unet->shape_ok(int(ctx->height), int(ctx->width)))
+    return fail(ctx, TRT_UNSUPPORTED, "denoising needs width % 128 == 0 and height % 8 == 0");
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  const uint64_t P = uint64_t(ctx->width) * ctx->height;
+  HIPC(ctx->dn_feat.ensure(P * kFeatures * 2), "alloc features");
+  HIPC(ctx->dn_denoised.ensure(P * 16), "alloc denoised");
+  hipEvent_t* ev = ctx->ev;
+  HIPC(hipEventRecord(ev[0], ctx->stream), "event");
+  dn_features(ctx->color.as<float4>(), ctx->albedo.as<float4>(), ctx->normal.as<float4>(), ctx->dn_feat.as<uint16_t>(),
+              P, ctx->stream);
+  HIPC(hipEventRecord(ev[1], ctx->stream), "event");
+  HIPC(ctx->unet->forward(ctx->dn_feat.as<uint16_t>(), 1, int(ctx->height), int(ctx->width), ctx->stream), "forward");
+  dn_output(ctx->unet->output(), ctx->dn_denoised.as<float4>(), P, ctx->stream);
+  HIPC(hipEventRecord(ev[2], ctx->stream), "event");
+  HIPC(hipStreamSynchronize(ctx->stream), "denoise");
+  if (st) {
+    (void)hipEventElapsedTime(&st->features_ms, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&st->network_ms, ev[1], ev[2]);
+    (void)hipEventElapsedTime(&st->total_ms, ev[0], ev[2]);
+  }
+  return TRT_OK;
+}
+
+trt_status trt_read_denoised(trt_context* ctx, float* rgba) {
+  if (!ctx || !rgba) return TRT_INVALID_ARG;
+  if (!ctx->dn_denoised.p || ctx->dn_denoised.bytes < size_t(ctx->width) * ctx->height * 16)
+    return fail(ctx, TRT_NOT_BUILT, "no denoised frame");
+  return read_pixels(ctx, ctx->dn_denoised, rgba);
+}
+
+// Test hooks on host arrays: features fp32 [N][H][W][16] (rounded to bf16 here),
+// output fp32 [N][H][W][32] (log space; channels >= 3 are padding).
+trt_status trt_denoiser_forward(trt_context* ctx, const float* features, uint32_t N, uint32_t H, uint32_t W,
+                                float* out) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (!features || !out || N == 0 || !ctx->unet->shape_ok(int(H), int(W)))
+    return fail(ctx, TRT_INVALID_ARG, "need width % 128 == 0, height % 8 == 0");
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  const uint64_t P = uint64_t(N) * H * W;
+  std::vector<uint16_t> hb(P * kFeatures);
+  for (uint64_t i = 0; i < hb.size(); ++i) hb[i] = host_f2bf(features[i]);
+  HIPC(ctx->dn_io.ensure(P * kFeatures * 2), "alloc");
+  HIPC(hipMemcpyAsync(ctx->dn_io.p, hb.data(), hb.size() * 2, hipMemcpyHostToDevice, ctx->stream), "upload");
+  HIPC(ctx->unet->forward(ctx->dn_io.as<uint16_t>(), int(N), int(H), int(W), ctx->stream), "forward");
+  HIPC(hipMemcpyAsync(out, ctx->unet->output(), P * 32 * 4, hipMemcpyDeviceToHost, ctx->stream), "read");
+  HIPC(hipStreamSynchronize(ctx->stream), "forward");
+  return TRT_OK;
+}
+
+// Post-activation output of convolution `index` (0..13) of the last forward, as floats.
+trt_status trt_denoiser_activation(trt_context* ctx, uint32_t index, float* out, uint64_t n) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  const size_t m = ctx->unet->activation_elems(int(index));
+  if (!out || m == 0 || n != m) return fail(ctx, TRT_INVALID_ARG, "index out of range or wrong element count");
+  std::vector<uint16_t> h(m);
+  HIPC(hipMemcpy(h.data(), ctx->unet->activation(int(index)), m * 2, hipMemcpyDeviceToHost), "read activation");
+  for (size_t i = 0; i < m; ++i) {
+    const uint32_t u = uint32_t(h[i]) << 16;
+    std::memcpy(&out[i], &u, 4);
+  }
+  return TRT_OK;
+}
+
+// Gradient of sum(dout * output) w.r.t. the parameters for the last trt_denoiser_forward.
+trt_status trt_denoiser_backward(trt_context* ctx, const float* dout, uint32_t N, uint32_t H, uint32_t W, float* grads) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (!dout || !grads) return TRT_INVALID_ARG;
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  const uint64_t P = uint64_t(N) * H * W;
+  DevBuf d;
+  HIPC(d.ensure(P * 32 * 4), "alloc");
+  HIPC(hipMemcpyAsync(d.p, dout, P * 32 * 4, hipMemcpyHostToDevice, ctx->stream), "upload");
+  HIPC(ctx->unet->backward(d.as<float>(), int(N), int(H), int(W), ctx->stream), "backward");
+  HIPC(ctx->unet->get_grads(grads, ctx->stream), "read grads");
   return TRT_OK;
 }
 

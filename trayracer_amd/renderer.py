@@ -207,6 +207,65 @@ class Renderer:
             acc += L[k]
         return acc * np.float32(1.0 / s.spp)
 
+    # -- denoiser ---------------------------------------------------------------------
+    def denoiser_init(self, seed=0):
+        self._check(self.lib.trt_denoiser_init(self.ctx, seed))
+
+    def denoiser_params(self):
+        n = int(self.lib.trt_denoiser_num_params())
+        out = np.empty(n, np.float32)
+        self._check(self.lib.trt_denoiser_get_params(self.ctx, _fp(out), n))
+        return out
+
+    def denoiser_set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(self.lib.trt_denoiser_set_params(self.ctx, _fp(w), w.size))
+
+    def denoiser_save(self, path):
+        self._check(self.lib.trt_denoiser_save(self.ctx, str(path).encode()))
+
+    def denoiser_load(self, path):
+        self._check(self.lib.trt_denoiser_load(self.ctx, str(path).encode()))
+
+    def denoiser_dataset(self, n, width, height):
+        self._check(self.lib.trt_denoiser_dataset(self.ctx, n, width, height))
+
+    def denoiser_capture(self, slot, kind):
+        self._check(self.lib.trt_denoiser_capture(self.ctx, slot, kind))
+
+    def denoiser_train(self, steps, batch=8, crop=128, lr=1e-3, seed=0):
+        losses = np.empty(steps, np.float32)
+        self._check(self.lib.trt_denoiser_train(self.ctx, steps, batch, crop, lr, seed, _fp(losses)))
+        return losses
+
+    def denoise(self):
+        st = _lib.TrtDenoiseStats()
+        self._check(self.lib.trt_denoise(self.ctx, ctypes.byref(st)))
+        return st.as_dict()
+
+    def denoised(self):
+        return self._pixels(self.lib.trt_read_denoised)
+
+    def denoiser_forward(self, features):
+        f = np.ascontiguousarray(features, np.float32)
+        N, H, W, C = f.shape
+        assert C == 16
+        out = np.empty((N, H, W, 32), np.float32)
+        self._check(self.lib.trt_denoiser_forward(self.ctx, _fp(f), N, H, W, _fp(out)))
+        return out
+
+    def denoiser_activation(self, index, shape):
+        out = np.empty(shape, np.float32)
+        self._check(self.lib.trt_denoiser_activation(self.ctx, index, _fp(out), out.size))
+        return out
+
+    def denoiser_backward(self, dout):
+        d = np.ascontiguousarray(dout, np.float32)
+        N, H, W, _ = d.shape
+        g = np.empty(int(self.lib.trt_denoiser_num_params()), np.float32)
+        self._check(self.lib.trt_denoiser_backward(self.ctx, _fp(d), N, H, W, _fp(g)))
+        return g
+
     # -- ray queries -----------------------------------------------------------------
     def trace_closest(self, rays):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
