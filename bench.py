@@ -28,6 +28,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="soup1m", choices=["soup1m", "cornell", "instanced"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-denoise", action="store_true",
+                    help="skip the time-to-relMSE measurement (denoiser training + convergence runs)")
+    ap.add_argument("--denoise-steps", type=int, default=4000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -48,7 +51,7 @@ def copy_calibrate():
 def make_scene(name):
     from trayracer_amd import scenes
     if name == "soup1m":
-        return scenes.soup(), "soup-1M-tris 1920x1080 8spp depth8"
+        return scenes.soup(), "soup-1M-tris 1920x1080 8spp depth8 (RR from depth 3)"
     if name == "cornell":
         return scenes.cornell(), "cornell-36-tris 256x256 16spp depth4"
     return scenes.instanced(), "instanced-1000x20480-tris 3840x2160 4spp depth6"
@@ -73,6 +76,39 @@ def cpu_baseline(scene, threads):
         "sample": f"same scene and camera at {W}x{H}, 1 spp, depth {r['max_depth']}: {rays} rays in {dt:.2f} s "
                   f"(oracle BVH build {build_s:.1f} s not timed)",
     }
+
+
+def denoise_metric(rend, steps):
+    """Time to relMSE 0.01 against a 4096-spp render, plain accumulation vs learned
+    denoising, on the Cornell box at 1920x1080 seen from its own (held-out) camera. The
+    denoiser is first trained on the GPU from frames the tracer renders (training time
+    reported separately); the instanced scene's view is reported beside it."""
+    import math
+
+    from trayracer_amd import convergence as C
+    from trayracer_amd import scenes
+
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    rend.denoiser_init(seed=0)
+    t_pairs = C.build_training_set(rend, views=(("cornell", 6), ("inst", 3), ("soup", 1)), log=log)
+    t_train, losses = C.train(rend, steps=steps, batch=8, crop=128, lr=1e-3)
+    log(f"denoiser trained: {steps} steps in {t_train:.1f} s, loss {float(losses[-50:].mean()):.5f}")
+    sc = scenes.cornell(width=1920, height=1080)
+    res = C.time_to_relmse(rend, sc, sc["camera"], max_plain_spp=4096, log=log)
+    inst = C.time_to_relmse(rend, scenes.instanced(), C.INST_CAMERA, max_plain_spp=256, max_denoise_spp=16, log=log)
+
+    def pick(r):
+        out = {k: r[k] for k in ("target", "plain_time_s", "plain_spp", "plain_how", "denoised_time_s",
+                                 "denoised_spp", "speedup", "reference_spp")}
+        out["plain_relmse_by_spp"] = {int(k): round(e, 5) for k, _, e in r["plain_curve"]}
+        out["denoised_relmse_by_spp"] = {int(k): round(e, 5) for k, _, e, _ in r["denoised_curve"]}
+        out["denoise_ms"] = round(r["denoised_curve"][-1][3], 3)
+        return out
+    return {"view": "cornell 1920x1080, its own camera (not a training view), depth 4",
+            "relmse": "mean (x - r)^2 / (r^2 + 1e-2) vs a 4096-spp render", **pick(res),
+            "train": {"pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1), "steps": steps, "batch": 8,
+                      "crop": 128, "final_loss": float(losses[-50:].mean())},
+            "instanced_view": pick(inst)}
 
 
 def main():
@@ -126,6 +162,9 @@ def main():
         "bytes_per_ray": round(bytes_per_ext, 1), "nodes_per_ray": round(tc["closest_nodes_per_ray"], 2),
         "tris_per_ray": round(tc["closest_tris_per_ray"], 2),
     }
+    dn = None
+    if not args.no_denoise:
+        dn = denoise_metric(rend, args.denoise_steps)
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -136,7 +175,9 @@ def main():
         "value": round(rays / wall / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": workload, "triangles": int(scene["positions"].shape[0]),
+        "config": {"workload": workload,
+                   "triangles": int(scene["positions"].shape[0]) * (1 if scene.get("instances") is None
+                                                                    else int(scene["instances"].shape[0])),
                    "rays_per_frame": rays // args.steps},
         "trace_mrays_s": round((ext_rays + sh_rays) / ((ext_ms + sh_ms) * 1e-3) / 1e6, 1),
         "extend_mrays_s": round(ext_rays / (ext_ms * 1e-3) / 1e6, 1),
@@ -146,7 +187,7 @@ def main():
                                "shadow": round(sh_ms / args.steps, 3)},
         "bvh": {"nodes": info["nodes"], "depth": info["depth"], "sah_cost": round(info["sah_cost"], 2),
                 "build_s": round(build_s, 2)},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "time_to_relmse": dn,
     }
     print(json.dumps(line), flush=True)
     rend.close()

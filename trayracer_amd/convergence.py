@@ -1,0 +1,146 @@
+"""Time to reach a relative MSE target, plain accumulation vs learned denoising.
+
+relMSE(x, r) = mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2), with r a
+high-spp render of the same view (4096 spp by default). Wall times are measured on
+the GPU with the scene resident: rendering k spp (plus, for the denoised path, one
+denoiser inference). The denoiser is trained first, on pairs (low-spp input, 1024-spp
+target) rendered by the tracer from the 1M-triangle soup, the instanced scene and the
+Cornell box under seeded random cameras; inputs use 1 to 8 spp so the network serves
+every sample count of the progressive render. Training time is reported separately.
+The evaluation view (the Cornell box's own camera) is not among the training views.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import scenes
+
+EPS = 1e-2
+
+
+def relmse(x, ref):
+    x = np.asarray(x, np.float64)[..., :3]
+    r = np.asarray(ref, np.float64)[..., :3]
+    return float(np.mean((x - r) ** 2 / (r * r + EPS)))
+
+
+def random_camera(rng, kind):
+    if kind == "cornell":  # in front of the open side of the box, looking inside
+        pos = np.array([rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(-1.6, -0.6)])
+        tgt = rng.uniform(0.3, 0.7, 3)
+        return dict(position=tuple(pos), target=tuple(tgt), up=(0.0, 1.0, 0.0),
+                    fov=math.radians(rng.uniform(30.0, 55.0)))
+    if kind == "soup":  # outside the unit cube, looking at a point inside it
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        pos = 0.5 + d * rng.uniform(0.95, 1.4)
+        tgt = rng.uniform(0.3, 0.7, 3)
+        fov = math.radians(rng.uniform(35.0, 60.0))
+    else:  # inside or near the 100^3 instance box
+        pos = rng.uniform(-20.0, 120.0, 3)
+        tgt = rng.uniform(25.0, 75.0, 3)
+        fov = math.radians(rng.uniform(45.0, 75.0))
+    return dict(position=tuple(pos), target=tuple(tgt), up=(0.0, 1.0, 0.0), fov=fov)
+
+
+INST_CAMERA = dict(position=(50.0, 50.0, -20.0), target=(50.0, 50.0, 50.0), up=(0.0, 1.0, 0.0),
+                   fov=math.radians(70.0))
+
+
+def _render_spp(rend, spp, seed, frame_spp):
+    """Accumulates `spp` samples (frames of at most frame_spp) from a reset; returns the
+    wall time in seconds."""
+    s = rend.settings
+    s.seed = seed
+    rend.Reset()
+    t0 = time.perf_counter()
+    left = spp
+    while left > 0:
+        s.spp = min(frame_spp, left)
+        rend.RenderFrame()
+        left -= s.spp
+    return time.perf_counter() - t0
+
+
+def build_training_set(rend, width=1920, height=1080, views=(("cornell", 6), ("inst", 4), ("soup", 2)),
+                       input_spp=(1, 2, 4, 8), target_spp=1024, seed=0, frame_spp=32, log=print):
+    """One slot per (view, input spp): soup views use 1 spp only (nearly black images)."""
+    rng = np.random.default_rng(seed)
+    plan = []
+    for kind, n in views:
+        levels = (1,) if kind == "soup" else tuple(input_spp)
+        plan += [(kind, levels)] * n
+    total = sum(len(levels) for _, levels in plan)
+    rend.denoiser_dataset(total, width, height)
+    slot, view = 0, 0
+    t0 = time.perf_counter()
+    current = None
+    for kind, levels in plan:
+        if kind != current:
+            sc = {"soup": scenes.soup, "inst": scenes.instanced, "cornell": scenes.cornell}[kind](
+                width=width, height=height)
+            rend.UpdateScene(sc)
+            current = kind
+        rend.SetCamera(random_camera(rng, kind))
+        _render_spp(rend, target_spp, seed=5000 + view, frame_spp=frame_spp)
+        first = slot
+        for k in levels:
+            rend.denoiser_capture(slot, 1)
+            slot += 1
+        for i, k in enumerate(levels):
+            _render_spp(rend, k, seed=1000 + 16 * view + i, frame_spp=frame_spp)
+            rend.denoiser_capture(first + i, 0)
+        view += 1
+    log(f"training set: {slot} pairs from {view} views in {time.perf_counter() - t0:.1f} s")
+    return time.perf_counter() - t0
+
+
+def train(rend, steps=600, batch=8, crop=128, lr=1e-3, seed=0):
+    t0 = time.perf_counter()
+    losses = rend.denoiser_train(steps, batch=batch, crop=crop, lr=lr, seed=seed)
+    return time.perf_counter() - t0, losses
+
+
+def time_to_relmse(rend, scene, camera, width=1920, height=1080, target=0.01, ref_spp=4096, max_plain_spp=1024,
+                   max_denoise_spp=64, frame_spp=32, log=print):
+    """Returns a dict with the plain and denoised times to `target` (seconds)."""
+    rend.UpdateScene(dict(scene, render=dict(scene["render"], width=width, height=height)))
+    rend.SetCamera(camera)
+    t_ref = _render_spp(rend, ref_spp, seed=999, frame_spp=frame_spp)
+    ref = rend.color()[..., :3].astype(np.float64)
+    log(f"reference: {ref_spp} spp in {t_ref:.1f} s")
+    plain = []
+    k = 1
+    while k <= max_plain_spp:
+        t = _render_spp(rend, k, seed=17, frame_spp=frame_spp)
+        e = relmse(rend.color(), ref)
+        plain.append((k, t, e))
+        if e <= target:
+            break
+        k *= 2
+    den = []
+    k = 1
+    while k <= max_denoise_spp:
+        t = _render_spp(rend, k, seed=17, frame_spp=frame_spp)
+        t0 = time.perf_counter()
+        st = rend.denoise()
+        td = time.perf_counter() - t0
+        e = relmse(rend.denoised(), ref)
+        den.append((k, t + td, e, st["total_ms"]))
+        if e <= target:
+            break
+        k *= 2
+    # plain: the sample count where relMSE crosses the target under the 1/spp law of
+    # Monte Carlo error from the last measured count (between the last two doublings
+    # when it crossed, beyond the budget otherwise), times the measured seconds per
+    # sample at that count. This is never more than the doubling search measured.
+    kp, tp, ep = plain[-1]
+    n_star = kp * ep / target
+    t_plain = tp / kp * n_star
+    plain_how = "interpolated (relMSE ~ 1/spp)" if ep <= target else "extrapolated (relMSE ~ 1/spp)"
+    kd, td_, ed, _ = den[-1]
+    t_den = td_ if ed <= target else None
+    return dict(target=target, plain_time_s=t_plain, plain_spp=n_star, plain_how=plain_how, denoised_time_s=t_den,
+                denoised_spp=kd if t_den is not None else None, speedup=(t_plain / t_den) if t_den else None,
+                plain_curve=plain, denoised_curve=den, reference_spp=ref_spp, reference_s=t_ref)

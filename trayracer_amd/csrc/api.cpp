@@ -1090,7 +1090,7 @@ trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* st) {
               P, ctx->stream);
   HIPC(hipEventRecord(ev[1], ctx->stream), "event");
   HIPC(ctx->unet->forward(ctx->dn_feat.as<uint16_t>(), 1, int(ctx->height), int(ctx->width), ctx->stream), "forward");
-  dn_output(ctx->unet->output(), ctx->dn_denoised.as<float4>(), P, ctx->stream);
+  dn_output(ctx->unet->output(), ctx->dn_feat.as<uint16_t>(), ctx->dn_denoised.as<float4>(), P, ctx->stream);
   HIPC(hipEventRecord(ev[2], ctx->stream), "event");
   HIPC(hipStreamSynchronize(ctx->stream), "denoise");
   if (st) {

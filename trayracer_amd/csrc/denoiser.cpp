@@ -65,7 +65,7 @@ hipError_t Unet::init(uint32_t seed, hipStream_t s) {
     const Conv& c = convs_[li];
     const int fan_cin = li == E0A ? 10 : c.cin;
     const int real_cout = li == OUT ? 3 : c.cout;
-    std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (9.f * fan_cin)) * (li == OUT ? 0.5f : 1.f));
+    std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (9.f * fan_cin)) * (li == OUT ? 0.1f : 1.f));
     for (int co = 0; co < real_cout; ++co)
       for (int t = 0; t < 9; ++t)
         for (int ci = 0; ci < (li == E0A ? 10 : c.cin); ++ci) h[c.woff + (size_t(co) * 9 + t) * c.cin + ci] = nd(rng);
@@ -231,7 +231,7 @@ hipError_t Unet::train_step(const uint16_t* x, const float* target, int N, int H
   hipError_t e = forward(x, N, H, W, s);
   if (e != hipSuccess) return e;
   const size_t P = size_t(N) * H * W;
-  const int nb = dn_loss(out_, target, P, dout_, lpart_, s);
+  const int nb = dn_loss(out_, x, target, P, dout_, lpart_, s);
   e = backward(dout_, N, H, W, s);
   if (e != hipSuccess) return e;
   ++t_;

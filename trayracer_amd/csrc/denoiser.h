@@ -33,15 +33,17 @@ void dn_features(const float4* color, const float4* albedo, const float4* normal
 void dn_target(const float4* color, float* out, size_t P, hipStream_t s);
 void dn_gather_crop(const uint16_t* feat, const float* tgt, int Hs, int Ws, const int4* crops, int B, int S,
                     uint16_t* fo, float* to, hipStream_t s);
-int dn_loss(const float* Y, const float* T, size_t P, float* dY, float* partial, hipStream_t s);
-void dn_output(const float* Y, float4* out, size_t P, hipStream_t s);
+int dn_loss(const float* Y, const uint16_t* X, const float* T, size_t P, float* dY, float* partial, hipStream_t s);
+void dn_output(const float* Y, const uint16_t* X, float4* out, size_t P, hipStream_t s);
 void dn_adam(float* w, float* m, float* v, const float* g, uint16_t* wb, const uint8_t* mask, size_t n, float lr,
              float b1, float b2, float c1, float c2, float eps, hipStream_t s);
 void dn_flip_weights(const uint16_t* w, uint16_t* wd, int cin, int cout, hipStream_t s);
 
 // Network: 4-level U-Net, 32/64/96/128 channels, 3x3 convolutions with ReLU, 2x2
 // average pooling, nearest upsampling + skip concatenation, a final 3x3 convolution to
-// 3 channels (padded to 32) in log(1 + radiance) space.
+// 3 channels (padded to 32). It predicts a log(1 + radiance) correction that is added
+// to the noisy input's log(1 + radiance), so an untrained (near-zero) output layer is
+// the identity and more input samples keep improving the result.
 class Unet {
  public:
   struct Conv {

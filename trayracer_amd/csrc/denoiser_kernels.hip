@@ -348,10 +348,12 @@ __global__ void k_gather_crop(const uint16_t* __restrict__ feat, const float* __
   to[dst * 3 + 2] = tgt[src * 3 + 2];
 }
 
-// L2 loss in log space on the first 3 of 32 output channels: partial sums per block and
-// dOut = 2 (y - t) / (3 P) (zero on padding channels)
-__global__ void k_loss(const float* __restrict__ Y, const float* __restrict__ T, size_t P, float* __restrict__ dY,
-                       float* __restrict__ partial) {
+// L2 loss in log space. The network predicts a correction added to the noisy input
+// (feature channels 0..2 = log(1 + radiance)): prediction = x + y over the first 3 of
+// 32 output channels. Partial sums per block and dOut = 2 (x + y - t) / (3 P) (zero
+// on padding channels).
+__global__ void k_loss(const float* __restrict__ Y, const uint16_t* __restrict__ X, const float* __restrict__ T,
+                       size_t P, float* __restrict__ dY, float* __restrict__ partial) {
   __shared__ float red[256];
   const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   float l = 0.f;
@@ -360,7 +362,7 @@ __global__ void k_loss(const float* __restrict__ Y, const float* __restrict__ T,
     for (int c = 0; c < 32; ++c) {
       float g = 0.f;
       if (c < 3) {
-        const float d = Y[p * 32 + c] - T[p * 3 + c];
+        const float d = (bf2f(X[p * 16 + c]) + Y[p * 32 + c]) - T[p * 3 + c];
         l += d * d;
         g = 2.f * d * inv;
       }
@@ -376,13 +378,14 @@ __global__ void k_loss(const float* __restrict__ Y, const float* __restrict__ T,
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// network output (log space) -> radiance
-__global__ void k_output(const float* __restrict__ Y, float4* __restrict__ out, size_t P) {
+// network output (log-space correction of the input) -> radiance
+__global__ void k_output(const float* __restrict__ Y, const uint16_t* __restrict__ X, float4* __restrict__ out,
+                         size_t P) {
   const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P) return;
-  const float r = expm1f(fmaxf(Y[p * 32 + 0], 0.f)), g = expm1f(fmaxf(Y[p * 32 + 1], 0.f)),
-              b = expm1f(fmaxf(Y[p * 32 + 2], 0.f));
-  out[p] = make_float4(r, g, b, 1.f);
+  float v[3];
+  for (int c = 0; c < 3; ++c) v[c] = expm1f(fmaxf(bf2f(X[p * 16 + c]) + Y[p * 32 + c], 0.f));
+  out[p] = make_float4(v[0], v[1], v[2], 1.f);
 }
 
 // Adam (Kingma & Ba 2015) on fp32 master weights; bf16 copy refreshed; `mask` zeroes
@@ -508,13 +511,13 @@ void dn_gather_crop(const uint16_t* feat, const float* tgt, int Hs, int Ws, cons
   hipLaunchKernelGGL(k_gather_crop, dim3(blocks_for(static_cast<size_t>(S) * S), B), dim3(256), 0, s, feat, tgt, Hs,
                      Ws, crops, S, fo, to);
 }
-int dn_loss(const float* Y, const float* T, size_t P, float* dY, float* partial, hipStream_t s) {
+int dn_loss(const float* Y, const uint16_t* X, const float* T, size_t P, float* dY, float* partial, hipStream_t s) {
   const unsigned nb = blocks_for(P);
-  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, Y, T, P, dY, partial);
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, Y, X, T, P, dY, partial);
   return static_cast<int>(nb);
 }
-void dn_output(const float* Y, float4* out, size_t P, hipStream_t s) {
-  hipLaunchKernelGGL(k_output, dim3(blocks_for(P)), dim3(256), 0, s, Y, out, P);
+void dn_output(const float* Y, const uint16_t* X, float4* out, size_t P, hipStream_t s) {
+  hipLaunchKernelGGL(k_output, dim3(blocks_for(P)), dim3(256), 0, s, Y, X, out, P);
 }
 void dn_adam(float* w, float* m, float* v, const float* g, uint16_t* wb, const uint8_t* mask, size_t n, float lr,
              float b1, float b2, float c1, float c2, float eps, hipStream_t s) {
