@@ -32,6 +32,8 @@ def parse():
                     help="skip the time-to-relMSE measurement (denoiser training + convergence runs)")
     ap.add_argument("--denoise-steps", type=int, default=4000)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--presplit", type=float, default=1.0,
+                    help="BVH spatial pre-splitting budget, extra references per triangle (0 = off)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
@@ -125,7 +127,7 @@ def main():
     scene, workload = make_scene(args.workload)
     rend = Renderer(0)
     t0 = time.perf_counter()
-    rend.UpdateScene(scene)
+    rend.UpdateScene(scene, presplit=args.presplit)
     build_s = time.perf_counter() - t0
     info = rend.bvh_info()
     for _ in range(args.warmup):
@@ -186,7 +188,7 @@ def main():
         "phase_ms_per_frame": {"extend": round(ext_ms / args.steps, 3), "shade": round(shade_ms / args.steps, 3),
                                "shadow": round(sh_ms / args.steps, 3)},
         "bvh": {"nodes": info["nodes"], "depth": info["depth"], "sah_cost": round(info["sah_cost"], 2),
-                "build_s": round(build_s, 2)},
+                "presplit": args.presplit, "references": info["references"], "build_s": round(build_s, 2)},
         "roofline": roofline, "cpu_baseline": cpu, "time_to_relmse": dn,
     }
     print(json.dumps(line), flush=True)

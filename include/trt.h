@@ -76,6 +76,9 @@ typedef struct trt_build_opts {
   int32_t bins;      /* binned-SAH bins per axis (32) */
   int32_t max_leaf;  /* primitives per leaf, 1..7 (4) */
   int32_t threads;   /* host build threads (0 = hardware concurrency) */
+  float presplit;    /* spatial pre-splitting budget: extra BVH references per triangle
+                        (0 = off; 1.0 roughly quarters the triangle tests on scenes of
+                        large overlapping triangles). Results do not depend on it. */
 } trt_build_opts;
 
 /* Render options (replaces KernelSettings, Tracer/Common/CommonStructs.h:145-153, plus the
@@ -121,6 +124,7 @@ typedef struct trt_bvh_info {
   uint64_t device_bytes;
   uint32_t two_level;
   uint32_t tlas_depth;
+  uint64_t references;  /* triangle references in the leaves (> triangles when pre-split) */
 } trt_bvh_info;
 
 typedef struct trt_pick_result {

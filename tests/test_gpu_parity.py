@@ -34,9 +34,10 @@ def _check_hits(h_gpu, i_gpu, h_ref, i_ref, label):
     assert np.array_equal(h_gpu[hit].view(np.uint32), h_ref[hit].view(np.uint32)), f"{label}: t/bary bits differ"
 
 
-def test_trace_cornell_exact(rend):
+@pytest.mark.parametrize("presplit", [0.0, 2.0])
+def test_trace_cornell_exact(rend, presplit):
     sc = scenes.cornell()
-    rend.UpdateScene(sc)
+    rend.UpdateScene(sc, presplit=presplit)
     orc = _oracle(sc)
     rays = random_rays(20000, seed=11, lo=0.02, hi=0.98)
     hg, ig = rend.trace_closest(rays)
@@ -53,11 +54,13 @@ def test_trace_cornell_exact(rend):
     assert np.array_equal(occ_g, occ_r)
 
 
-def test_trace_soup_exact(rend):
+@pytest.mark.parametrize("presplit", [0.0, 1.0, 3.0])
+def test_trace_soup_exact(rend, presplit):
     sc = scenes.soup(n_tris=200_000, seed=5, width=64, height=64)
-    rend.UpdateScene(sc)
+    rend.UpdateScene(sc, presplit=presplit)
     info = rend.bvh_info()
     assert info["depth"] >= 10
+    assert (info["references"] > 200_000) == (presplit > 0)
     orc = _oracle(sc)
     rays = random_rays(50000, seed=12, lo=-0.3, hi=1.3, tmin=1e-5)
     hg, ig = rend.trace_closest(rays)
@@ -102,9 +105,10 @@ def test_path_radiance_cornell_rr_deep(rend):
     _compare_paths(rend.path_radiance(), ref["path_rgb"], "cornell rr")
 
 
-def test_path_radiance_soup(rend):
+@pytest.mark.parametrize("presplit", [0.0, 1.0])
+def test_path_radiance_soup(rend, presplit):
     sc = scenes.soup(n_tris=50_000, seed=3, width=48, height=32, spp=2, max_depth=8)
-    rend.UpdateScene(sc)
+    rend.UpdateScene(sc, presplit=presplit)
     rend.RenderFrame()
     ref = _oracle(sc).render()
     _compare_paths(rend.path_radiance(), ref["path_rgb"], "soup")

@@ -38,7 +38,8 @@ class TrtCamera(ctypes.Structure):
 
 
 class TrtBuildOpts(ctypes.Structure):
-    _fields_ = [("bins", ctypes.c_int32), ("max_leaf", ctypes.c_int32), ("threads", ctypes.c_int32)]
+    _fields_ = [("bins", ctypes.c_int32), ("max_leaf", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("presplit", ctypes.c_float)]
 
 
 class TrtRenderOpts(ctypes.Structure):
@@ -68,6 +69,7 @@ class TrtBvhInfo(ctypes.Structure):
         ("nodes", ctypes.c_uint64), ("leaves", ctypes.c_uint64), ("depth", ctypes.c_uint32),
         ("stack_entries", ctypes.c_uint32), ("sah_cost", ctypes.c_double), ("build_ms", ctypes.c_double),
         ("device_bytes", ctypes.c_uint64), ("two_level", ctypes.c_uint32), ("tlas_depth", ctypes.c_uint32),
+        ("references", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -109,6 +109,7 @@ def time_to_relmse(rend, scene, camera, width=1920, height=1080, target=0.01, re
     rend.SetCamera(camera)
     t_ref = _render_spp(rend, ref_spp, seed=999, frame_spp=frame_spp)
     ref = rend.color()[..., :3].astype(np.float64)
+    rend.denoise()  # untimed warm-up: first launches load the kernels
     log(f"reference: {ref_spp} spp in {t_ref:.1f} s")
     plain = []
     k = 1

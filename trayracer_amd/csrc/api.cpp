@@ -188,10 +188,27 @@ void tri_boxes(const float* pos, uint32_t first, uint32_t n, std::vector<float>&
   }
 }
 
+// Builds the BVH over a mesh's triangles pos[first .. first+n), pre-split when
+// presplit > 0; ref_prim receives the mesh-local triangle of each build primitive.
+BuildResult build_mesh(const float* pos, uint32_t first, uint32_t n, const trt_build_opts& o, int threads,
+                       std::vector<uint32_t>& ref_prim) {
+  std::vector<float> box;
+  if (o.presplit > 0.f) {
+    presplit_triangles(pos + size_t(first) * 9, n, o.presplit, box, ref_prim);
+  } else {
+    tri_boxes(pos, first, n, box);
+    ref_prim.resize(n);
+    for (uint32_t i = 0; i < n; ++i) ref_prim[i] = i;
+  }
+  return build_binned_sah(box.data(), uint32_t(ref_prim.size()), o.bins, o.max_leaf, threads);
+}
+
 // Appends a mesh BVH in device layout: node indices offset by node_base, leaf
-// triangle ranges offset by tri_base; traversal triangles get mesh-local prim ids.
-void append_blas(const float* pos, uint32_t tri_offset, const BuildResult& b, std::vector<DevNode2>& nodes,
-                 std::vector<float>& tris) {
+// triangle ranges offset by tri_base; traversal triangles get mesh-local prim ids
+// (ref_prim maps build primitives to triangles; a pre-split triangle is stored once
+// per leaf that references it).
+void append_blas(const float* pos, uint32_t tri_offset, const BuildResult& b, const std::vector<uint32_t>& ref_prim,
+                 std::vector<DevNode2>& nodes, std::vector<float>& tris) {
   std::vector<DevNode2> dn = to_device_layout(b);
   const int32_t node_base = int32_t(nodes.size());
   const uint32_t tri_base = uint32_t(tris.size() / 12);
@@ -208,7 +225,7 @@ void append_blas(const float* pos, uint32_t tri_offset, const BuildResult& b, st
     nodes.push_back(n);
   }
   for (uint32_t k = 0; k < b.prim_order.size(); ++k) {
-    const uint32_t local = b.prim_order[k];
+    const uint32_t local = ref_prim[b.prim_order[k]];
     const float* p = pos + size_t(tri_offset + local) * 9;
     float rec[12] = {p[0], p[1], p[2], 0.f, p[3], p[4], p[5], 0.f, p[6], p[7], p[8], 0.f};
     std::memcpy(&rec[3], &local, 4);
@@ -499,9 +516,11 @@ trt_status trt_set_instances(trt_context* ctx, const trt_instance* inst, uint32_
 
 trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   if (!ctx) return TRT_INVALID_ARG;
-  trt_build_opts o = opts ? *opts : trt_build_opts{32, 4, 0};
+  trt_build_opts o = opts ? *opts : trt_build_opts{32, 4, 0, 0.f};
   if (o.bins < 2 || o.bins > 256 || o.max_leaf < 1 || o.max_leaf > 7)
     return fail(ctx, TRT_INVALID_ARG, "bins must be in [2,256] and max_leaf in [1,7]");
+  if (!(o.presplit >= 0.f && o.presplit <= 8.f))
+    return fail(ctx, TRT_INVALID_ARG, "presplit must be in [0, 8]");
   int threads = o.threads > 0 ? o.threads : int(std::thread::hardware_concurrency());
   if (threads <= 0) threads = 1;
   const uint32_t n_tris = uint32_t(ctx->mat_ids.size());
@@ -534,17 +553,17 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
 
   std::vector<DevNode2> nodes;
   std::vector<float> tris;
-  std::vector<float> box;
   std::vector<Instance> inst_orig, inst_tlas;
   std::vector<DevNode2> tlas_nodes;
   ctx->two_level = !ctx->instances_in.empty();
-  uint64_t n_nodes = 0, n_leaves = 0;
+  uint64_t n_nodes = 0, n_leaves = 0, n_refs = 0;
   double sah = 0.0;
   int depth = 0, tlas_depth = 0;
   if (!ctx->two_level) {
-    tri_boxes(ctx->pos.data(), 0, n_tris, box);
-    BuildResult b = build_binned_sah(box.data(), n_tris, o.bins, o.max_leaf, threads);
-    append_blas(ctx->pos.data(), 0, b, nodes, tris);
+    std::vector<uint32_t> ref_prim;
+    BuildResult b = build_mesh(ctx->pos.data(), 0, n_tris, o, threads, ref_prim);
+    append_blas(ctx->pos.data(), 0, b, ref_prim, nodes, tris);
+    n_refs = ref_prim.size();
     // single-level: prim ids are global triangle indices (tri_offset 0)
     n_nodes = b.nodes.size();
     n_leaves = b.n_leaves;
@@ -555,10 +574,11 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
     std::vector<float> mesh_lo(ctx->meshes.size() * 3), mesh_hi(ctx->meshes.size() * 3);
     for (size_t mi = 0; mi < ctx->meshes.size(); ++mi) {
       const Mesh& m = ctx->meshes[mi];
-      tri_boxes(ctx->pos.data(), m.tri_offset, m.n_tris, box);
-      BuildResult b = build_binned_sah(box.data(), m.n_tris, o.bins, o.max_leaf, threads);
+      std::vector<uint32_t> ref_prim;
+      BuildResult b = build_mesh(ctx->pos.data(), m.tri_offset, m.n_tris, o, threads, ref_prim);
       blas_root[mi] = uint32_t(nodes.size());
-      append_blas(ctx->pos.data(), m.tri_offset, b, nodes, tris);
+      append_blas(ctx->pos.data(), m.tri_offset, b, ref_prim, nodes, tris);
+      n_refs += ref_prim.size();
       n_nodes += b.nodes.size();
       n_leaves += b.n_leaves;
       depth = std::max(depth, b.depth);
@@ -658,6 +678,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   ctx->info.device_bytes = nodes.size() * sizeof(DevNode2) + tris.size() * 4 + tlas_nodes.size() * sizeof(DevNode2);
   ctx->info.two_level = ctx->two_level ? 1 : 0;
   ctx->info.tlas_depth = uint32_t(tlas_depth);
+  ctx->info.references = n_refs;
   ctx->built = true;
   ctx->sample_count = 0;
   return TRT_OK;

@@ -219,7 +219,138 @@ void finish_stats(BuildResult& r) {
   }
 }
 
+struct Poly {
+  int n = 0;
+  double v[12][3];
+};
+
+void clip_plane(const Poly& in, int axis, double c, bool keep_below, Poly& out) {
+  out.n = 0;
+  for (int i = 0; i < in.n; ++i) {
+    const double* a = in.v[i];
+    const double* b = in.v[(i + 1) % in.n];
+    const double da = keep_below ? c - a[axis] : a[axis] - c;
+    const double db = keep_below ? c - b[axis] : b[axis] - c;
+    if (da >= 0 && out.n < 12) {
+      std::memcpy(out.v[out.n++], a, sizeof(double) * 3);
+    }
+    if ((da < 0) != (db < 0) && out.n < 12) {
+      const double t = da / (da - db);
+      double* o = out.v[out.n++];
+      for (int k = 0; k < 3; ++k) o[k] = a[k] + t * (b[k] - a[k]);
+      o[axis] = c;
+    }
+  }
+}
+
+void poly_box(const Poly& p, double lo[3], double hi[3]) {
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = 1e300;
+    hi[k] = -1e300;
+  }
+  for (int i = 0; i < p.n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], p.v[i][k]);
+      hi[k] = std::max(hi[k], p.v[i][k]);
+    }
+}
+
+double box_area(const double lo[3], const double hi[3]) {
+  const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+  return 2.0 * (dx * dy + dy * dz + dz * dx);
+}
+
+void emit_box(const double lo[3], const double hi[3], const float* tlo, const float* thi, uint32_t prim,
+              std::vector<float>& box, std::vector<uint32_t>& refs) {
+  float b[6];
+  for (int k = 0; k < 3; ++k) {
+    // outward rounding, clamped to the triangle's own box
+    float l = float(lo[k]), h = float(hi[k]);
+    if (double(l) > lo[k]) l = std::nextafter(l, -std::numeric_limits<float>::infinity());
+    if (double(h) < hi[k]) h = std::nextafter(h, std::numeric_limits<float>::infinity());
+    b[k] = std::max(l, tlo[k]);
+    b[3 + k] = std::min(h, thi[k]);
+  }
+  box.insert(box.end(), b, b + 6);
+  refs.push_back(prim);
+}
+
+void split_rec(const Poly& p, int splits, const float* tlo, const float* thi, uint32_t prim, std::vector<float>& box,
+               std::vector<uint32_t>& refs) {
+  double lo[3], hi[3];
+  poly_box(p, lo, hi);
+  if (splits <= 0 || p.n < 3) {
+    emit_box(lo, hi, tlo, thi, prim, box, refs);
+    return;
+  }
+  int axis = 0;
+  for (int k = 1; k < 3; ++k)
+    if (hi[k] - lo[k] > hi[axis] - lo[axis]) axis = k;
+  const double c = 0.5 * (lo[axis] + hi[axis]);
+  Poly a, b;
+  clip_plane(p, axis, c, true, a);
+  clip_plane(p, axis, c, false, b);
+  if (a.n < 3 || b.n < 3) {
+    emit_box(lo, hi, tlo, thi, prim, box, refs);
+    return;
+  }
+  double alo[3], ahi[3], blo[3], bhi[3];
+  poly_box(a, alo, ahi);
+  poly_box(b, blo, bhi);
+  const double aa = box_area(alo, ahi), ab = box_area(blo, bhi);
+  const int rest = splits - 1;
+  const int sa = aa + ab > 0 ? int(std::lround(rest * aa / (aa + ab))) : rest / 2;
+  split_rec(a, sa, tlo, thi, prim, box, refs);
+  split_rec(b, rest - sa, tlo, thi, prim, box, refs);
+}
+
 }  // namespace
+
+void presplit_triangles(const float* pos, uint32_t n, float beta, std::vector<float>& ref_box,
+                        std::vector<uint32_t>& ref_prim) {
+  ref_box.clear();
+  ref_prim.clear();
+  std::vector<double> excess(n);
+  double total = 0.0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* p = pos + size_t(i) * 9;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min<double>(p[k], std::min<double>(p[3 + k], p[6 + k]));
+      hi[k] = std::max<double>(p[k], std::max<double>(p[3 + k], p[6 + k]));
+    }
+    const double e1[3] = {double(p[3]) - p[0], double(p[4]) - p[1], double(p[5]) - p[2]};
+    const double e2[3] = {double(p[6]) - p[0], double(p[7]) - p[1], double(p[8]) - p[2]};
+    const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                 cz = e1[0] * e2[1] - e1[1] * e2[0];
+    const double tri_area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    excess[i] = std::max(0.0, box_area(lo, hi) - 2.0 * tri_area);
+    total += excess[i];
+  }
+  const double budget = double(beta) * n;
+  ref_box.reserve(size_t((n + budget) * 6 * 1.05));
+  ref_prim.reserve(size_t((n + budget) * 1.05));
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* p = pos + size_t(i) * 9;
+    float tlo[3], thi[3];
+    for (int k = 0; k < 3; ++k) {
+      tlo[k] = std::min(p[k], std::min(p[3 + k], p[6 + k]));
+      thi[k] = std::max(p[k], std::max(p[3 + k], p[6 + k]));
+    }
+    const int splits = total > 0 ? int(budget * excess[i] / total) : 0;
+    if (splits == 0) {
+      ref_box.insert(ref_box.end(), tlo, tlo + 3);
+      ref_box.insert(ref_box.end(), thi, thi + 3);
+      ref_prim.push_back(i);
+      continue;
+    }
+    Poly poly;
+    poly.n = 3;
+    for (int v = 0; v < 3; ++v)
+      for (int k = 0; k < 3; ++k) poly.v[v][k] = p[v * 3 + k];
+    split_rec(poly, splits, tlo, thi, i, ref_box, ref_prim);
+  }
+}
 
 BuildResult build_binned_sah(const float* prim_box_data, uint32_t n, int bins, int max_leaf, int n_threads) {
   BuildResult res;

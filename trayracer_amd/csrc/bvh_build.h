@@ -30,6 +30,17 @@ struct BuildResult {
 // prim_box: 6 floats per primitive (lo xyz, hi xyz).
 BuildResult build_binned_sah(const float* prim_box, uint32_t n, int bins, int max_leaf, int n_threads);
 
+// Spatial pre-splitting of triangles (after Karras & Aila 2013, "Fast parallel
+// construction of high-quality bounding volume hierarchies", section 5): a budget of
+// beta * n extra references is distributed over the triangles in proportion to how much
+// their bounding box exceeds their own area; each selected triangle is cut recursively
+// at the spatial median of its piece's longest axis (Sutherland-Hodgman clipping in
+// double precision), giving several tighter references to the same triangle. Boxes are
+// rounded outward to float so their union always covers the triangle.
+// Outputs 6 floats per reference and the triangle index of each reference.
+void presplit_triangles(const float* tri_pos, uint32_t n, float beta, std::vector<float>& ref_box,
+                        std::vector<uint32_t>& ref_prim);
+
 // Device layout: one node stores both children's boxes (64 bytes, four float4 rows):
 //   row0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
 //   row1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)

@@ -84,8 +84,11 @@ class Renderer:
             raise TrtError(st, self.lib.trt_last_error(self.ctx).decode())
 
     # -- scene -----------------------------------------------------------------------
-    def UpdateScene(self, scene, bins=32, max_leaf=4, threads=0):
-        """Uploads a scene dict (see trayracer_amd.scenes) and builds its BVH(s)."""
+    def UpdateScene(self, scene, bins=32, max_leaf=4, threads=0, presplit=1.0):
+        """Uploads a scene dict (see trayracer_amd.scenes) and builds its BVH(s).
+
+        presplit: spatial pre-splitting budget in extra references per triangle (0 = off).
+        """
         lib, ctx = self.lib, self.ctx
         self._check(lib.trt_clear_scene(ctx))
         mats = np.ascontiguousarray(scene["materials"], np.float32)
@@ -107,7 +110,7 @@ class Renderer:
             ci = (_lib.TrtInstance * len(inst))()
             ctypes.memmove(ci, inst.ctypes.data, inst.nbytes)
             self._check(lib.trt_set_instances(ctx, ci, len(inst)))
-        bo = _lib.TrtBuildOpts(bins, max_leaf, threads)
+        bo = _lib.TrtBuildOpts(bins, max_leaf, threads, presplit)
         self._check(lib.trt_build(ctx, ctypes.byref(bo)))
         if "camera" in scene:
             self.SetCamera(scene["camera"])
