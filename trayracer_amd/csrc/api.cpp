@@ -107,6 +107,8 @@ struct trt_context {
   DevBuf counters;
   DevBuf scratch_rays, scratch_hits, scratch_inst, scratch_occ;
   uint64_t path_capacity = 0;
+  uint64_t queue_slots = 0;  // kSegs * seg_cap slots allocated per queue
+  uint32_t seg_cap = 0;      // slots per queue segment for the current frame size
 
   hipEvent_t ev[4 + 4 * kMaxDepth] = {};
   int grid_trace = 0, grid_shade = 0;
@@ -248,18 +250,24 @@ trt_status ensure_frame(trt_context* ctx, uint32_t W, uint32_t H, uint32_t spp) 
     ctx->height = H;
     ctx->sample_count = 0;
   }
-  if (n > ctx->path_capacity) {
+  // queues: kSegs segments of seg_cap(n) slots (slack for uneven segment fill)
+  ctx->seg_cap = seg_cap(n);
+  const uint64_t slots = uint64_t(kSegs) * ctx->seg_cap;
+  if (slots > ctx->queue_slots) {
     for (int i = 0; i < 2; ++i) {
-      HIPC(ctx->q_o[i].ensure(n * 16), "alloc queue");
-      HIPC(ctx->q_d[i].ensure(n * 16), "alloc queue");
-      HIPC(ctx->q_t[i].ensure(n * 16), "alloc queue");
-      HIPC(ctx->q_m[i].ensure(n * 4), "alloc queue");
+      HIPC(ctx->q_o[i].ensure(slots * 16), "alloc queue");
+      HIPC(ctx->q_d[i].ensure(slots * 16), "alloc queue");
+      HIPC(ctx->q_t[i].ensure(slots * 16), "alloc queue");
+      HIPC(ctx->q_m[i].ensure(slots * 4), "alloc queue");
     }
-    HIPC(ctx->h_t.ensure(n * 16), "alloc hits");
-    HIPC(ctx->h_i.ensure(n * 4), "alloc hits");
-    HIPC(ctx->s_o.ensure(n * 16), "alloc shadow queue");
-    HIPC(ctx->s_d.ensure(n * 16), "alloc shadow queue");
-    HIPC(ctx->s_c.ensure(n * 16), "alloc shadow queue");
+    HIPC(ctx->h_t.ensure(slots * 16), "alloc hits");
+    HIPC(ctx->h_i.ensure(slots * 4), "alloc hits");
+    HIPC(ctx->s_o.ensure(slots * 16), "alloc shadow queue");
+    HIPC(ctx->s_d.ensure(slots * 16), "alloc shadow queue");
+    HIPC(ctx->s_c.ensure(slots * 16), "alloc shadow queue");
+    ctx->queue_slots = slots;
+  }
+  if (n > ctx->path_capacity) {
     HIPC(ctx->radiance.ensure(n * 16), "alloc radiance");
     ctx->path_capacity = n;
   }
@@ -341,27 +349,32 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
   hipEvent_t* ev = ctx->ev;
   const uint32_t D = std::min<uint32_t>(o.max_depth, kMaxDepth);
 
+  const uint32_t cap = ctx->seg_cap;
   launch_zero_counters(ctr, stm);
   HIPC(hipEventRecord(ev[0], stm), "event");
-  launch_raygen(rp, n, q[0], fb.radiance, ctr, stm);
+  launch_raygen(rp, n, cap, q[0], fb.radiance, ctr, stm);
   HIPC(hipEventRecord(ev[1], stm), "event");
   for (uint32_t d = 0; d < D; ++d) {
     const int cur = int(d & 1u), nxt = cur ^ 1;
-    launch_extend(acc, q[cur], hb, &ctr[kCtrExt + d], &ctr[kCtrWorkExt + d], ctr, ctx->grid_trace, stats, stm);
+    const int di = int(d);
+    launch_extend(acc, q[cur], hb, SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkExt, di), cap}, ctr,
+                  ctx->grid_trace, stats, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 0], stm), "event");
-    launch_shade(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq, fb, &ctr[kCtrExt + d],
-                 &ctr[kCtrWorkShade + d], &ctr[kCtrExt + d + 1], &ctr[kCtrShadow + d], d, ctx->grid_shade, stm);
+    launch_shade(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq, fb,
+                 SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkShade, di), cap},
+                 ctr + seg_ctr(kSegExt, di + 1), ctr + seg_ctr(kSegShadow, di), ctr, d, ctx->grid_shade, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 1], stm), "event");
-    launch_shadow(acc, sq, fb.radiance, &ctr[kCtrShadow + d], &ctr[kCtrWorkShadow + d], ctr, ctx->grid_trace, stats,
-                  stm);
+    launch_shadow(acc, sq, fb.radiance, SegIn{ctr + seg_ctr(kSegShadow, di), ctr + seg_ctr(kSegWorkShadow, di), cap},
+                  ctr, ctx->grid_trace, stats, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 2], stm), "event");
   }
   if (accumulate) launch_finalize(rp, fb.radiance, fb, ctx->sample_count == 0 ? 1u : 0u, stm);
   HIPC(hipEventRecord(ev[2], stm), "event");
+  launch_sum_counts(ctr, cap, stm);
   HIPC(hipGetLastError(), "kernel launch");
   HIPC(hipStreamSynchronize(stm), "frame");
 
-  uint32_t hc[kNumCounters];
+  uint32_t hc[kCtrSegBase];
   HIPC(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost), "read counters");
   ctx->last_paths = n;
   ctx->has_frame = true;
@@ -398,7 +411,8 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
     st->sample_count = ctx->sample_count;
     st->overflow = hc[kCtrOverflow];
   }
-  if (hc[kCtrOverflow]) return fail(ctx, TRT_DEVICE, "traversal stack overflow");
+  if (hc[kCtrOverflow] & 1u) return fail(ctx, TRT_DEVICE, "traversal stack overflow");
+  if (hc[kCtrOverflow] & 2u) return fail(ctx, TRT_DEVICE, "ray queue overflow");
   (void)stats;
   return TRT_OK;
 }
@@ -891,8 +905,8 @@ trt_status trt_backward(trt_context* ctx, const trt_render_opts* o, const float*
   HIPC(ctx->a_pend.ensure(n * 64), "alloc adjoint");
   HIPC(ctx->a_key.ensure(R * 4), "alloc adjoint");
   HIPC(ctx->a_g.ensure(R * 4 * kNumParams), "alloc adjoint");
-  HIPC(ctx->a_sga.ensure(n * 16), "alloc adjoint");
-  HIPC(ctx->a_sgb.ensure(n * 4), "alloc adjoint");
+  HIPC(ctx->a_sga.ensure(ctx->queue_slots * 16), "alloc adjoint");  // per shadow-queue slot
+  HIPC(ctx->a_sgb.ensure(ctx->queue_slots * 4), "alloc adjoint");
   HIPC(ctx->a_dldi.ensure(uint64_t(W) * H * 12), "alloc adjoint");
   HIPC(ctx->a_partial.ensure(size_t(kGradBlocks) * kGradMatsPerPass * kNumParams * 4), "alloc adjoint");
   HIPC(ctx->a_out.ensure(size_t(nm) * kNumParams * 4 + 16), "alloc adjoint");
@@ -923,14 +937,19 @@ trt_status trt_backward(trt_context* ctx, const trt_render_opts* o, const float*
   HitBuf hb{ctx->h_t.as<float4>(), ctx->h_i.as<uint32_t>()};
   ShadowQueue sq{ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>()};
   hipStream_t stm = ctx->stream;
+  const uint32_t cap = ctx->seg_cap;
   launch_zero_counters(ctr, stm);
-  launch_raygen_adj(rp, uint32_t(n), q[0], ctx->radiance.as<float4>(), ab, ctr, stm);
+  launch_raygen_adj(rp, uint32_t(n), cap, q[0], ctx->radiance.as<float4>(), ab, ctr, stm);
   for (uint32_t d = 0; d < D; ++d) {
     const int cur = int(d & 1u), nxt = cur ^ 1;
-    launch_extend(acc, q[cur], hb, &ctr[kCtrExt + d], &ctr[kCtrWorkExt + d], ctr, ctx->grid_trace, false, stm);
-    launch_shade_adj(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq, ab, &ctr[kCtrExt + d],
-                     &ctr[kCtrWorkShade + d], &ctr[kCtrExt + d + 1], &ctr[kCtrShadow + d], d, ctx->grid_shade, stm);
-    launch_shadow_adj(acc, sq, ab, &ctr[kCtrShadow + d], &ctr[kCtrWorkShadow + d], ctr, d, ctx->grid_trace, stm);
+    const int di = int(d);
+    launch_extend(acc, q[cur], hb, SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkExt, di), cap}, ctr,
+                  ctx->grid_trace, false, stm);
+    launch_shade_adj(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq, ab,
+                     SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkShade, di), cap},
+                     ctr + seg_ctr(kSegExt, di + 1), ctr + seg_ctr(kSegShadow, di), ctr, d, ctx->grid_shade, stm);
+    launch_shadow_adj(acc, sq, ab, SegIn{ctr + seg_ctr(kSegShadow, di), ctr + seg_ctr(kSegWorkShadow, di), cap}, ctr,
+                      d, ctx->grid_trace, stm);
   }
   for (uint32_t lo = 0; lo < nm; lo += kGradMatsPerPass) {
     const uint32_t cnt = std::min(kGradMatsPerPass, nm - lo);

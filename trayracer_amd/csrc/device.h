@@ -41,22 +41,56 @@ struct DevAccel {
   int two_level;
 };
 
+// Queues are split into kSegs segments of `cap` slots each (slot = segment * cap + i).
+// Every segment has its own append and work-fetch counters, each on its own 128-byte
+// line: returning atomics on one line serialise at about 88 per microsecond on MI355X,
+// so a single shared counter caps a queue at ~5.6 G rays/s (one atomic per 64 rays);
+// spread over 32 lines they stop mattering. Producer waves append to their home
+// segment (wave index mod kSegs) and move on to the next segment when it is full;
+// consumer waves fetch 64-ray batches from their home segment first, then from the
+// others. Capacity is 1.5x the path count, so appends always find room.
+constexpr uint32_t kSegs = 32;
+constexpr uint32_t kLine = 32;  // uint32 words per 128-byte line
+
 // Counters in device memory; zeroed at the start of every frame by a kernel.
 // Layout (uint32 indices):
-//   [kCtrExt + d]   rays in the extension queue of depth d (d = 0: primary rays)
-//   [kCtrShadow + d] shadow rays emitted at depth d
-//   [kCtrWorkExt + d], [kCtrWorkShade + d], [kCtrWorkShadow + d]: work-fetch cursors
+//   [kCtrExt + d]    total rays in the extension queue of depth d (d = 0: primary rays),
+//   [kCtrShadow + d] total shadow rays emitted at depth d; both written at the end of a
+//                    frame by k_sum_counts from the per-segment counters
+//   [kCtrTrace]      work cursor of the standalone trace entry points
+//   [kCtrOverflow]   bit 0: traversal stack overflow; bit 1: queue append failed
 //   [kCtrStats + k]  traversal statistics (node visits, triangle tests: 64-bit pairs)
+//   [seg_ctr(kind, d) + s * kLine] per-segment counters, kinds below
 constexpr int kMaxDepth = 32;
 constexpr int kCtrExt = 0;
 constexpr int kCtrShadow = 32;
-constexpr int kCtrWorkExt = 64;
-constexpr int kCtrWorkShade = 96;
-constexpr int kCtrWorkShadow = 128;
-constexpr int kCtrTrace = 160;  // work cursor of the standalone trace entry points
+constexpr int kCtrTrace = 160;
 constexpr int kCtrOverflow = 161;
 constexpr int kCtrStats = 164;  // 4 x uint64: closest nodes, closest tris, any nodes, any tris
-constexpr int kNumCounters = 192;
+constexpr int kCtrSegBase = 256;
+enum SegKind : int { kSegExt = 0, kSegShadow = 1, kSegWorkExt = 2, kSegWorkShade = 3, kSegWorkShadow = 4 };
+constexpr int kNumSegKinds = 5;
+constexpr int kSegDepths = kMaxDepth + 1;  // the last shade may address depth kMaxDepth
+constexpr int kNumCounters = kCtrSegBase + kNumSegKinds * kSegDepths * int(kSegs * kLine);
+constexpr int seg_ctr(int kind, int depth) { return kCtrSegBase + (kind * kSegDepths + depth) * int(kSegs * kLine); }
+
+// Slots per segment for n paths (a multiple of 64, kSegs * cap >= 1.5 n), and the
+// primary-ray layout: path p sits in segment p / prim_per_seg(n) (prim_per_seg <= cap).
+inline uint32_t seg_cap(uint64_t n) {
+  const uint64_t c = (3 * n + 2 * kSegs - 1) / (2 * kSegs);
+  return static_cast<uint32_t>((c + 63) / 64 * 64);
+}
+inline uint32_t prim_per_seg(uint64_t n) {
+  const uint64_t c = (n + kSegs - 1) / kSegs;
+  return static_cast<uint32_t>((c + 63) / 64 * 64);
+}
+
+// A segmented queue's counters as seen by a kernel.
+struct SegIn {
+  const uint32_t* cnt;  // kSegs lines: items appended per segment (may exceed cap)
+  uint32_t* work;       // kSegs lines: fetch cursors
+  uint32_t cap;
+};
 
 struct FrameBuffers {
   float4* radiance;   // per path (rgb)
@@ -86,27 +120,28 @@ constexpr uint32_t kGradMatsPerPass = 512;
 // Resident 256-thread blocks per CU of the traversal kernels for this accel's LDS stack.
 int trace_blocks_per_cu(const DevAccel& acc);
 
-// Kernel launchers (kernels.hip). All run on `stream`.
+// Kernel launchers (kernels.hip). All run on `stream`. `cap` is the per-segment
+// capacity of every queue; out_cnt / shadow_cnt point at kSegs counter lines.
 void launch_zero_counters(uint32_t* ctr, hipStream_t stream);
-void launch_raygen(const RenderParams& rp, uint32_t n_paths, RayQueue q, float4* radiance, uint32_t* ctr,
+void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, float4* radiance,
+                   uint32_t* ctr, hipStream_t stream);
+void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr, int grid, bool stats,
                    hipStream_t stream);
-void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, const uint32_t* count, uint32_t* work,
-                   uint32_t* ctr, int grid, bool stats, hipStream_t stream);
 void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
-                  HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, const uint32_t* count, uint32_t* work,
-                  uint32_t* next_count, uint32_t* shadow_count, uint32_t depth, int grid, hipStream_t stream);
-void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, const uint32_t* count, uint32_t* work,
-                   uint32_t* ctr, int grid, bool stats, hipStream_t stream);
+                  HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, SegIn in, uint32_t* next_cnt,
+                  uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream);
+void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,
+                   bool stats, hipStream_t stream);
+void launch_sum_counts(uint32_t* ctr, uint32_t cap, hipStream_t stream);
 void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
                      hipStream_t stream);
-void launch_raygen_adj(const RenderParams& rp, uint32_t n_paths, RayQueue q, const float4* radiance,
+void launch_raygen_adj(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, const float4* radiance,
                        const AdjBuffers& ab, uint32_t* ctr, hipStream_t stream);
 void launch_shade_adj(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
-                      HitBuf h, RayQueue qn, ShadowQueue sq, const AdjBuffers& ab, const uint32_t* count,
-                      uint32_t* work, uint32_t* next_count, uint32_t* shadow_count, uint32_t depth, int grid,
-                      hipStream_t stream);
-void launch_shadow_adj(const DevAccel& acc, ShadowQueue sq, const AdjBuffers& ab, const uint32_t* count,
-                       uint32_t* work, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream);
+                      HitBuf h, RayQueue qn, ShadowQueue sq, const AdjBuffers& ab, SegIn in, uint32_t* next_cnt,
+                      uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream);
+void launch_shadow_adj(const DevAccel& acc, ShadowQueue sq, const AdjBuffers& ab, SegIn in, uint32_t* ctr,
+                       uint32_t depth, int grid, hipStream_t stream);
 // partial: kGradBlocks * n_mat * 5 floats; out: n_mat * 5 floats (device)
 void launch_grad_reduce(const AdjBuffers& ab, uint32_t mat_lo, uint32_t n_mat, float* partial, float* out,
                         hipStream_t stream);

@@ -13,6 +13,9 @@
 //  * Shade: the shared integrator step (trt/integrator.h) per ray; extension and
 //    shadow rays are appended with one atomic per wave (ballot + mbcnt prefix) rather
 //    than one atomic per ray as in the reference (PathTracing.h:135-141, 168-173).
+//  * Queues are segmented (device.h: kSegs segments, each with its own counters on
+//    separate cache lines) so that fetch and append atomics spread over 32 lines.
+//  * RayGen writes path p to a fixed slot (no atomics): the primary queue is dense.
 //  * No host synchronisation between bounces: queue sizes stay in device counters.
 //  * Radiance is accumulated per path (one writer per path per kernel) and reduced
 //    per pixel in a fixed sample order in Finalize, so images are deterministic.
@@ -35,13 +38,6 @@ __device__ __forceinline__ uint32_t prefix_count(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
 }
 
-// Reserves `cnt` slots at *ctr for the wave (called by all lanes, uniform control flow).
-__device__ __forceinline__ uint32_t wave_reserve(uint32_t* ctr, uint32_t cnt) {
-  uint32_t base = 0;
-  if (lane_id() == 0 && cnt) base = atomicAdd(ctr, cnt);
-  return __shfl(base, 0);
-}
-
 __device__ __forceinline__ uint32_t wave_fetch(uint32_t* work) {
   uint32_t base = 0;
   if (lane_id() == 0) base = atomicAdd(work, 64u);
@@ -53,6 +49,74 @@ __device__ __forceinline__ void wave_add_stat(unsigned long long* dst, uint32_t 
   uint32_t s = v;
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
   if (lane_id() == 0 && s) atomicAdd(dst, static_cast<unsigned long long>(s));
+}
+
+// Fetches the next batch of up to 64 items of a segmented queue for the wave, starting
+// with segment `home`. Returns false when every segment is exhausted. `n_seg` holds,
+// in lane s < kSegs, the item count of segment s (constant during the kernel); `done`
+// collects segments found exhausted (work cursors only grow, so they stay exhausted):
+// every wave makes at most kSegs failed attempts, so the loop always ends.
+__device__ __forceinline__ bool seg_fetch(const SegIn& in, uint32_t n_seg, uint32_t home, uint64_t& done,
+                                          uint32_t& slot0, uint32_t& n_valid) {
+  const uint32_t lane = lane_id();
+  for (;;) {
+    uint32_t rem = 0;
+    if (lane < kSegs && !((done >> lane) & 1ull)) {
+      const uint32_t w = __hip_atomic_load(&in.work[lane * kLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      rem = w < n_seg ? n_seg - w : 0u;
+    }
+    const uint64_t m = __ballot(rem > 0u);
+    if (m == 0ull) return false;
+    const uint64_t rot = ((m >> home) | (m << (kSegs - home))) & ((1ull << kSegs) - 1ull);
+    uint32_t sgm = home + static_cast<uint32_t>(__builtin_ctzll(rot));
+    if (sgm >= kSegs) sgm -= kSegs;
+    uint32_t b = 0;
+    if (lane == 0) b = atomicAdd(&in.work[sgm * kLine], 64u);
+    b = __shfl(b, 0);
+    const uint32_t ns = __shfl(n_seg, static_cast<int>(sgm));
+    if (b < ns) {
+      slot0 = sgm * in.cap + b;
+      n_valid = ns - b < 64u ? ns - b : 64u;
+      return true;
+    }
+    done |= 1ull << sgm;
+  }
+}
+
+// Per-lane item count of segment `lane` (lanes >= kSegs: 0).
+__device__ __forceinline__ uint32_t seg_counts(const SegIn& in) {
+  const uint32_t lane = lane_id();
+  if (lane >= kSegs) return 0u;
+  const uint32_t c = in.cnt[lane * kLine];
+  return c < in.cap ? c : in.cap;
+}
+
+// Home segment of the calling wave.
+__device__ __forceinline__ uint32_t wave_home() { return (blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kSegs; }
+
+// Appends the lanes in `mask` to a segmented queue; returns this lane's slot (kNoHit
+// for lanes not in the mask). Starts at segment `home` and moves on while segments are
+// full; a partly full segment takes what fits. Total capacity (kSegs * cap) is at
+// least the number of items ever appended per queue, so the loop places everything
+// within kSegs attempts; otherwise bit 1 of *overflow is set.
+__device__ __forceinline__ uint32_t seg_append(uint32_t* cnt, uint32_t cap, uint32_t home, uint64_t mask,
+                                               uint32_t* overflow) {
+  const uint32_t lane = lane_id();
+  const uint32_t need = static_cast<uint32_t>(__popcll(mask));
+  const uint32_t rank = prefix_count(mask);
+  const bool mine = (mask >> lane) & 1ull;
+  uint32_t slot = kNoHit, done = 0, sgm = home;
+  for (uint32_t t = 0; t < kSegs && done < need; ++t) {
+    uint32_t b = 0;
+    if (lane == 0) b = atomicAdd(&cnt[sgm * kLine], need - done);
+    b = __shfl(b, 0);
+    const uint32_t fit = b < cap ? (need - done < cap - b ? need - done : cap - b) : 0u;
+    if (mine && rank >= done && rank < done + fit) slot = sgm * cap + b + (rank - done);
+    done += fit;
+    sgm = sgm + 1 == kSegs ? 0u : sgm + 1;
+  }
+  if (done < need && lane == 0) atomicOr(overflow, 2u);
+  return slot;
 }
 
 struct TraceOut {
@@ -146,12 +210,108 @@ __device__ __forceinline__ bool traverse_bvh(const DevNode2* __restrict__ nodes,
   return false;
 }
 
+// Sentinel of an exhausted traversal (never a leaf code: leaf first indices are < 2^28).
+constexpr int32_t kTravDone = static_cast<int32_t>(0x80000000u);
+
+// Single-level BVH2 traversal in the "speculative while-while" form of Aila & Laine
+// (2009, "Understanding the efficiency of ray traversal on GPUs"): lanes walk inner
+// nodes until every lane of the wave holds a leaf (a lane that meets its first leaf
+// parks it and keeps walking), then the wave intersects leaves together. Triangle
+// tests are the costly step; this keeps them from running for a handful of lanes at
+// every iteration. The result does not depend on the visiting order (closest hit under
+// the (t, inst, prim) order, or any hit), so it equals traverse_bvh's.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool traverse_ww(const DevNode2* __restrict__ nodes, const float4* __restrict__ tris, V3 o,
+                                            V3 d, float tmin, uint32_t* stk, int cap, TraceOut& r,
+                                            uint32_t* overflow) {
+  const RayTri rt = ray_tri_setup(o, d);
+  const V3 inv = safe_inv(d);
+  int sp = 0;
+  int32_t node = 0;  // next node: inner index, leaf code, or kTravDone
+  int32_t leaf = 0;  // parked leaf code (0 = none)
+  auto pop = [&]() -> int32_t {
+    if (sp == 0) return kTravDone;
+    --sp;
+    return static_cast<int32_t>(stk[sp * 64]);
+  };
+  while (node != kTravDone || leaf != 0) {
+    // inner nodes
+    while (node >= 0) {
+      float4 r0, r1, r2;
+      int4 c;
+      load_node(nodes, node, r0, r1, r2, c);
+      if (STATS) r.nodes++;
+      float t0, t1;
+      const bool h0 = slab(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, o, inv, tmin, r.t, t0);
+      const bool h1 = slab(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, o, inv, tmin, r.t, t1);
+      if (h0 && h1) {
+        int32_t nearc = c.x, farc = c.y;
+        if (t1 < t0) {
+          nearc = c.y;
+          farc = c.x;
+        }
+        if (sp < cap) {
+          stk[sp * 64] = static_cast<uint32_t>(farc);
+          ++sp;
+        } else {
+          atomicOr(overflow, 1u);
+        }
+        node = nearc;
+      } else if (h0) {
+        node = c.x;
+      } else if (h1) {
+        node = c.y;
+      } else {
+        node = pop();
+      }
+      if (node < 0 && node != kTravDone && leaf == 0) {  // park the first leaf, keep walking
+        leaf = node;
+        node = pop();
+      }
+      if (__all(leaf != 0)) break;
+    }
+    // leaves
+    while (leaf != 0) {
+      const uint32_t code = ~static_cast<uint32_t>(leaf);
+      const uint32_t first = code >> 3, cnt = code & 7u;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const float4* tp = tris + static_cast<size_t>(first + k) * 3u;
+        const float4 a = tp[0], b = tp[1], cc = tp[2];
+        if (STATS) r.tests++;
+        TriHit th;
+        if (intersect_triangle(rt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), tmin, r.t, th)) {
+          const uint32_t prim = __float_as_uint(a.w);
+          if (ANY) {
+            r.t = th.t;
+            r.prim = prim;
+            r.inst = kNoInstance;
+            return true;
+          }
+          if (th.t < r.t || prim < r.prim) {
+            r.t = th.t;
+            r.b1 = th.b1;
+            r.b2 = th.b2;
+            r.prim = prim;
+            r.inst = kNoInstance;
+          }
+        }
+      }
+      if (node < 0 && node != kTravDone) {
+        leaf = node;
+        node = pop();
+      } else {
+        leaf = 0;
+      }
+    }
+  }
+  return false;
+}
+
 // Full trace (single or two-level). r.t must hold tmax on entry.
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trace_ray(const DevAccel& acc, V3 o, V3 d, float tmin, uint32_t* stk,
                                           TraceOut& r, uint32_t* overflow) {
-  if (!acc.two_level) return traverse_bvh<ANY, STATS>(acc.nodes, acc.tris, 0, o, d, tmin, kNoInstance, stk, 0,
-                                                      acc.stack_cap, r, overflow);
+  if (!acc.two_level) return traverse_ww<ANY, STATS>(acc.nodes, acc.tris, o, d, tmin, stk, acc.stack_cap, r, overflow);
   // top level: leaf ranges index the instance array in TLAS build order
   const V3 inv = safe_inv(d);
   int sp = 0;
@@ -209,46 +369,74 @@ __device__ __forceinline__ bool trace_ray(const DevAccel& acc, V3 o, V3 d, float
 }
 
 // ------------------------------------------------------------------------------------
-__global__ void k_zero_counters(uint32_t* ctr) {
-  const int i = threadIdx.x;
-  if (i < kNumCounters) ctr[i] = 0u;
+__global__ __launch_bounds__(kBlock) void k_zero_counters(uint32_t* ctr) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < static_cast<uint32_t>(kNumCounters)) ctr[i] = 0u;
 }
 
-__global__ __launch_bounds__(kBlock) void k_raygen(RenderParams rp, uint32_t n_paths, RayQueue q, float4* radiance,
-                                                    uint32_t* ctr) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  PathState ps;
-  bool alive = false;
-  if (p < n_paths) {
-    alive = raygen(rp, p, ps);
-    radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+// Totals per depth of the extension and shadow queues (lane = depth, two waves).
+__global__ __launch_bounds__(64) void k_sum_counts(uint32_t* ctr, uint32_t cap) {
+  const uint32_t t = threadIdx.x;
+  if (t >= 2u * kMaxDepth) return;
+  const uint32_t kind = t < kMaxDepth ? kSegExt : kSegShadow, d = t % kMaxDepth;
+  const uint32_t* c = ctr + seg_ctr(static_cast<int>(kind), static_cast<int>(d));
+  uint32_t total = 0;
+  for (uint32_t sgm = 0; sgm < kSegs; ++sgm) {
+    const uint32_t v = c[sgm * kLine];
+    total += v < cap ? v : cap;
   }
-  const uint64_t m = __ballot(alive);
-  const uint32_t base = wave_reserve(&ctr[kCtrExt + 0], static_cast<uint32_t>(__popcll(m)));
-  if (alive) {
-    const uint32_t s = base + prefix_count(m);
-    q.o_tmin[s] = make_float4(ps.o.x, ps.o.y, ps.o.z, rp.eps);
-    q.d_tmax[s] = make_float4(ps.d.x, ps.d.y, ps.d.z, 1e30f);
-    q.t_pdf[s] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.pdf);
-    q.meta[s] = (p << 2) | ps.flags;
+  ctr[(kind == kSegExt ? kCtrExt : kCtrShadow) + d] = total;
+}
+
+// Primary-ray slot of path p: segment p / per, in order (no atomics).
+__device__ __forceinline__ uint32_t primary_slot(uint32_t p, uint32_t per, uint32_t cap) {
+  return (p / per) * cap + p % per;
+}
+
+// Writes the primary-queue counts (one lane per segment).
+__device__ __forceinline__ void primary_counts(uint32_t n_paths, uint32_t per, uint32_t* cnt) {
+  if (blockIdx.x == 0 && threadIdx.x < kSegs) {
+    const uint32_t lo = threadIdx.x * per;
+    cnt[threadIdx.x * kLine] = n_paths > lo ? (n_paths - lo < per ? n_paths - lo : per) : 0u;
+  }
+}
+
+// Stores a primary ray; a camera sample with zero weight is stored with tmax = -1, so
+// the traversal rejects it at the root and Shade skips it.
+__device__ __forceinline__ void store_primary(const RayQueue& q, uint32_t s, uint32_t p, const PathState& ps,
+                                              bool alive, float eps) {
+  q.o_tmin[s] = make_float4(ps.o.x, ps.o.y, ps.o.z, eps);
+  q.d_tmax[s] = make_float4(ps.d.x, ps.d.y, ps.d.z, alive ? 1e30f : -1.f);
+  q.t_pdf[s] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.pdf);
+  q.meta[s] = (p << 2) | ps.flags;
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen(RenderParams rp, uint32_t n_paths, uint32_t per, uint32_t cap,
+                                                    RayQueue q, float4* radiance, uint32_t* ctr) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  primary_counts(n_paths, per, ctr + seg_ctr(kSegExt, 0));
+  if (p < n_paths) {
+    PathState ps;
+    const bool alive = raygen(rp, p, ps);
+    radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    store_primary(q, primary_slot(p, per, cap), p, ps, alive, rp.eps);
   }
 }
 
 template <bool STATS>
-__global__ __launch_bounds__(kBlock) void k_extend(DevAccel acc, RayQueue q, HitBuf h, const uint32_t* count_p,
-                                                    uint32_t* work, uint32_t* ctr) {
+__global__ __launch_bounds__(kBlock) void k_extend(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
   const uint32_t lane = lane_id();
   uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
-  const uint32_t count = *count_p;
-  for (;;) {
-    const uint32_t base = wave_fetch(work);
-    if (base >= count) break;
-    const uint32_t slot = base + lane;
+  const uint32_t n_seg = seg_counts(in), home = wave_home();
+  uint64_t done = 0;
+  uint32_t slot0, n_valid;
+  while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
+    const uint32_t slot = slot0 + lane;
     TraceOut r;
     r.nodes = 0;
     r.tests = 0;
-    if (slot < count) {
+    if (lane < n_valid) {
       const float4 a = q.o_tmin[slot];
       const float4 b = q.d_tmax[slot];
       r.t = b.w;
@@ -268,20 +456,20 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevAccel acc, RayQueue q, Hit
 }
 
 template <bool STATS>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevAccel acc, ShadowQueue sq, float4* radiance,
-                                                    const uint32_t* count_p, uint32_t* work, uint32_t* ctr) {
+__global__ __launch_bounds__(kBlock) void k_shadow(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
+                                                    uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
   const uint32_t lane = lane_id();
   uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
-  const uint32_t count = *count_p;
-  for (;;) {
-    const uint32_t base = wave_fetch(work);
-    if (base >= count) break;
-    const uint32_t slot = base + lane;
+  const uint32_t n_seg = seg_counts(in), home = wave_home();
+  uint64_t done = 0;
+  uint32_t slot0, n_valid;
+  while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
+    const uint32_t slot = slot0 + lane;
     TraceOut r;
     r.nodes = 0;
     r.tests = 0;
-    if (slot < count) {
+    if (lane < n_valid) {
       const float4 a = sq.o_tmin[slot];
       const float4 b = sq.d_tmax[slot];
       r.t = b.w;
@@ -308,23 +496,23 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevAccel acc, ShadowQueue sq,
 
 __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* sampling_mats, RenderParams rp,
                                                    RayQueue q, HitBuf h, RayQueue qn, ShadowQueue sq,
-                                                   FrameBuffers fb, const uint32_t* count_p, uint32_t* work,
-                                                   uint32_t* next_count, uint32_t* shadow_count, uint32_t depth,
+                                                   FrameBuffers fb, SegIn in, uint32_t* next_cnt,
+                                                   uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth,
                                                    uint32_t two_level) {
   const uint32_t lane = lane_id();
-  const uint32_t count = *count_p;
   const uint32_t npix = rp.width * rp.height;
-  for (;;) {
-    const uint32_t base = wave_fetch(work);
-    if (base >= count) break;
-    const uint32_t slot = base + lane;
+  const uint32_t n_seg = seg_counts(in), home = wave_home();
+  uint64_t done = 0;
+  uint32_t slot0, n_valid;
+  while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
+    const uint32_t slot = slot0 + lane;
     ShadeOut out;
     out.has_ext = 0;
     out.has_shadow = 0;
     uint32_t path = 0;
-    if (slot < count) {
+    const float4 b = lane < n_valid ? q.d_tmax[slot] : make_float4(0.f, 0.f, 0.f, -1.f);
+    if (b.w >= 0.f) {
       const float4 a = q.o_tmin[slot];
-      const float4 b = q.d_tmax[slot];
       const float4 tp = q.t_pdf[slot];
       const uint32_t meta = q.meta[slot];
       const float4 hh = h.thit[slot];
@@ -359,17 +547,17 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* 
     const bool shw = out.has_shadow != 0;
     const uint64_t me = __ballot(ext);
     const uint64_t ms = __ballot(shw);
-    const uint32_t be = wave_reserve(next_count, static_cast<uint32_t>(__popcll(me)));
-    const uint32_t bs = wave_reserve(shadow_count, static_cast<uint32_t>(__popcll(ms)));
-    if (ext) {
-      const uint32_t s = be + prefix_count(me);
+    const uint32_t se = seg_append(next_cnt, in.cap, home, me, &ctr[kCtrOverflow]);
+    const uint32_t ss = seg_append(shadow_cnt, in.cap, home, ms, &ctr[kCtrOverflow]);
+    if (ext && se != kNoHit) {
+      const uint32_t s = se;
       qn.o_tmin[s] = make_float4(out.next.o.x, out.next.o.y, out.next.o.z, rp.eps);
       qn.d_tmax[s] = make_float4(out.next.d.x, out.next.d.y, out.next.d.z, 1e30f);
       qn.t_pdf[s] = make_float4(out.next.T.x, out.next.T.y, out.next.T.z, out.next.pdf);
       qn.meta[s] = (path << 2) | out.next.flags;
     }
-    if (shw) {
-      const uint32_t s = bs + prefix_count(ms);
+    if (shw && ss != kNoHit) {
+      const uint32_t s = ss;
       sq.o_tmin[s] = make_float4(out.s_o.x, out.s_o.y, out.s_o.z, rp.eps);
       sq.d_tmax[s] = make_float4(out.s_d.x, out.s_d.y, out.s_d.z, out.s_tmax);
       sq.contrib[s] = make_float4(out.s_contrib.x, out.s_contrib.y, out.s_contrib.z, __uint_as_float(path));
@@ -437,50 +625,43 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevAccel acc, const float4* ra
 // Adjoint (path-replay backpropagation). Gradient records live in fixed slots
 // r = path * D + depth, so their values never depend on queue order; the reduction
 // below sums them in slot order.
-__global__ __launch_bounds__(kBlock) void k_raygen_adj(RenderParams rp, uint32_t n_paths, RayQueue q,
-                                                        const float4* radiance, AdjBuffers ab, uint32_t* ctr) {
+__global__ __launch_bounds__(kBlock) void k_raygen_adj(RenderParams rp, uint32_t n_paths, uint32_t per, uint32_t cap,
+                                                        RayQueue q, const float4* radiance, AdjBuffers ab,
+                                                        uint32_t* ctr) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  PathState ps;
-  bool alive = false;
+  primary_counts(n_paths, per, ctr + seg_ctr(kSegExt, 0));
   if (p < n_paths) {
-    alive = raygen(rp, p, ps);
+    PathState ps;
+    const bool alive = raygen(rp, p, ps);
     ab.lrem[p] = radiance[p];
     for (uint32_t d = 0; d < ab.depth; ++d) ab.rec_key[static_cast<size_t>(p) * ab.depth + d] = kNoHit;
     ab.pend[static_cast<size_t>(p) * 4 + 3] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kNoHit));
-  }
-  const uint64_t m = __ballot(alive);
-  const uint32_t base = wave_reserve(&ctr[kCtrExt + 0], static_cast<uint32_t>(__popcll(m)));
-  if (alive) {
-    const uint32_t s = base + prefix_count(m);
-    q.o_tmin[s] = make_float4(ps.o.x, ps.o.y, ps.o.z, rp.eps);
-    q.d_tmax[s] = make_float4(ps.d.x, ps.d.y, ps.d.z, 1e30f);
-    q.t_pdf[s] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.pdf);
-    q.meta[s] = (p << 2) | ps.flags;
+    store_primary(q, primary_slot(p, per, cap), p, ps, alive, rp.eps);
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneView sv, const Material* sampling_mats, RenderParams rp,
                                                        RayQueue q, HitBuf h, RayQueue qn, ShadowQueue sq,
-                                                       AdjBuffers ab, const uint32_t* count_p, uint32_t* work,
-                                                       uint32_t* next_count, uint32_t* shadow_count, uint32_t depth,
+                                                       AdjBuffers ab, SegIn in, uint32_t* next_cnt,
+                                                       uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth,
                                                        uint32_t two_level) {
   const uint32_t lane = lane_id();
-  const uint32_t count = *count_p;
   const uint32_t npix = rp.width * rp.height;
   const float inv_spp = 1.f / static_cast<float>(rp.spp);
   const size_t R = ab.n_records;
-  for (;;) {
-    const uint32_t base = wave_fetch(work);
-    if (base >= count) break;
-    const uint32_t slot = base + lane;
+  const uint32_t n_seg = seg_counts(in), home = wave_home();
+  uint64_t done = 0;
+  uint32_t slot0, n_valid;
+  while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
+    const uint32_t slot = slot0 + lane;
     ShadeOut out;
     AdjOut adj;
     out.has_ext = 0;
     out.has_shadow = 0;
     uint32_t path = 0;
-    if (slot < count) {
+    const float4 b = lane < n_valid ? q.d_tmax[slot] : make_float4(0.f, 0.f, 0.f, -1.f);
+    if (b.w >= 0.f) {
       const float4 a = q.o_tmin[slot];
-      const float4 b = q.d_tmax[slot];
       const float4 tp = q.t_pdf[slot];
       const uint32_t meta = q.meta[slot];
       const float4 hh = h.thit[slot];
@@ -545,17 +726,17 @@ __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneView sv, const Materi
     const bool shw = out.has_shadow != 0;
     const uint64_t me = __ballot(ext);
     const uint64_t ms = __ballot(shw);
-    const uint32_t be = wave_reserve(next_count, static_cast<uint32_t>(__popcll(me)));
-    const uint32_t bs = wave_reserve(shadow_count, static_cast<uint32_t>(__popcll(ms)));
-    if (ext) {
-      const uint32_t s = be + prefix_count(me);
+    const uint32_t se = seg_append(next_cnt, in.cap, home, me, &ctr[kCtrOverflow]);
+    const uint32_t ss = seg_append(shadow_cnt, in.cap, home, ms, &ctr[kCtrOverflow]);
+    if (ext && se != kNoHit) {
+      const uint32_t s = se;
       qn.o_tmin[s] = make_float4(out.next.o.x, out.next.o.y, out.next.o.z, rp.eps);
       qn.d_tmax[s] = make_float4(out.next.d.x, out.next.d.y, out.next.d.z, 1e30f);
       qn.t_pdf[s] = make_float4(out.next.T.x, out.next.T.y, out.next.T.z, out.next.pdf);
       qn.meta[s] = (path << 2) | out.next.flags;
     }
-    if (shw) {
-      const uint32_t s = bs + prefix_count(ms);
+    if (shw && ss != kNoHit) {
+      const uint32_t s = ss;
       sq.o_tmin[s] = make_float4(out.s_o.x, out.s_o.y, out.s_o.z, rp.eps);
       sq.d_tmax[s] = make_float4(out.s_d.x, out.s_d.y, out.s_d.z, out.s_tmax);
       sq.contrib[s] = make_float4(out.s_contrib.x, out.s_contrib.y, out.s_contrib.z, __uint_as_float(path));
@@ -565,19 +746,18 @@ __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneView sv, const Materi
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_shadow_adj(DevAccel acc, ShadowQueue sq, AdjBuffers ab,
-                                                        const uint32_t* count_p, uint32_t* work, uint32_t* ctr,
-                                                        uint32_t depth) {
+__global__ __launch_bounds__(kBlock) void k_shadow_adj(DevAccel acc, ShadowQueue sq, AdjBuffers ab, SegIn in,
+                                                        uint32_t* ctr, uint32_t depth) {
   extern __shared__ uint32_t lds[];
   const uint32_t lane = lane_id();
   uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
-  const uint32_t count = *count_p;
   const size_t R = ab.n_records;
-  for (;;) {
-    const uint32_t base = wave_fetch(work);
-    if (base >= count) break;
-    const uint32_t slot = base + lane;
-    if (slot < count) {
+  const uint32_t n_seg = seg_counts(in), home = wave_home();
+  uint64_t done = 0;
+  uint32_t slot0, n_valid;
+  while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
+    const uint32_t slot = slot0 + lane;
+    if (lane < n_valid) {
       const float4 a = sq.o_tmin[slot];
       const float4 b = sq.d_tmax[slot];
       TraceOut r;
@@ -683,40 +863,43 @@ int trace_blocks_per_cu(const DevAccel& acc) {
 }
 
 void launch_zero_counters(uint32_t* ctr, hipStream_t stream) {
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(kNumCounters), 0, stream, ctr);
+  hipLaunchKernelGGL(k_zero_counters, dim3((kNumCounters + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, ctr);
 }
 
-void launch_raygen(const RenderParams& rp, uint32_t n_paths, RayQueue q, float4* radiance, uint32_t* ctr,
-                   hipStream_t stream) {
+void launch_sum_counts(uint32_t* ctr, uint32_t cap, hipStream_t stream) {
+  hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, stream, ctr, cap);
+}
+
+void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, float4* radiance,
+                   uint32_t* ctr, hipStream_t stream) {
   const uint32_t blocks = (n_paths + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_raygen, dim3(blocks), dim3(kBlock), 0, stream, rp, n_paths, q, radiance, ctr);
+  hipLaunchKernelGGL(k_raygen, dim3(blocks), dim3(kBlock), 0, stream, rp, n_paths, prim_per_seg(n_paths), cap, q,
+                     radiance, ctr);
 }
 
-void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, const uint32_t* count, uint32_t* work,
-                   uint32_t* ctr, int grid, bool stats, hipStream_t stream) {
+void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr, int grid, bool stats,
+                   hipStream_t stream) {
   if (stats)
-    hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, count,
-                       work, ctr);
+    hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in, ctr);
   else
-    hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, count,
-                       work, ctr);
+    hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in, ctr);
 }
 
 void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
-                  HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, const uint32_t* count, uint32_t* work,
-                  uint32_t* next_count, uint32_t* shadow_count, uint32_t depth, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, fb, count,
-                     work, next_count, shadow_count, depth, sv.instances ? 1u : 0u);
+                  HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, SegIn in, uint32_t* next_cnt,
+                  uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, fb, in,
+                     next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u);
 }
 
-void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, const uint32_t* count, uint32_t* work,
-                   uint32_t* ctr, int grid, bool stats, hipStream_t stream) {
+void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,
+                   bool stats, hipStream_t stream) {
   if (stats)
-    hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance,
-                       count, work, ctr);
+    hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance, in,
+                       ctr);
   else
-    hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance,
-                       count, work, ctr);
+    hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance, in,
+                       ctr);
 }
 
 void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
@@ -733,24 +916,24 @@ void launch_trace(const DevAccel& acc, const float4* rays, uint32_t n, float4* h
 }
 
 
-void launch_raygen_adj(const RenderParams& rp, uint32_t n_paths, RayQueue q, const float4* radiance,
+void launch_raygen_adj(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, const float4* radiance,
                        const AdjBuffers& ab, uint32_t* ctr, hipStream_t stream) {
   const uint32_t blocks = (n_paths + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_raygen_adj, dim3(blocks), dim3(kBlock), 0, stream, rp, n_paths, q, radiance, ab, ctr);
+  hipLaunchKernelGGL(k_raygen_adj, dim3(blocks), dim3(kBlock), 0, stream, rp, n_paths, prim_per_seg(n_paths), cap, q,
+                     radiance, ab, ctr);
 }
 
 void launch_shade_adj(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
-                      HitBuf h, RayQueue qn, ShadowQueue sq, const AdjBuffers& ab, const uint32_t* count,
-                      uint32_t* work, uint32_t* next_count, uint32_t* shadow_count, uint32_t depth, int grid,
-                      hipStream_t stream) {
-  hipLaunchKernelGGL(k_shade_adj, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, ab, count,
-                     work, next_count, shadow_count, depth, sv.instances ? 1u : 0u);
+                      HitBuf h, RayQueue qn, ShadowQueue sq, const AdjBuffers& ab, SegIn in, uint32_t* next_cnt,
+                      uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_shade_adj, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, ab, in,
+                     next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u);
 }
 
-void launch_shadow_adj(const DevAccel& acc, ShadowQueue sq, const AdjBuffers& ab, const uint32_t* count,
-                       uint32_t* work, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k_shadow_adj, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, ab, count, work,
-                     ctr, depth);
+void launch_shadow_adj(const DevAccel& acc, ShadowQueue sq, const AdjBuffers& ab, SegIn in, uint32_t* ctr,
+                       uint32_t depth, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_shadow_adj, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, ab, in, ctr,
+                     depth);
 }
 
 void launch_grad_reduce(const AdjBuffers& ab, uint32_t mat_lo, uint32_t n_mat, float* partial, float* out,

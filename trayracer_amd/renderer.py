@@ -84,7 +84,7 @@ class Renderer:
             raise TrtError(st, self.lib.trt_last_error(self.ctx).decode())
 
     # -- scene -----------------------------------------------------------------------
-    def UpdateScene(self, scene, bins=32, max_leaf=4, threads=0, presplit=1.0):
+    def UpdateScene(self, scene, bins=32, max_leaf=4, threads=0, presplit=3.0):
         """Uploads a scene dict (see trayracer_amd.scenes) and builds its BVH(s).
 
         presplit: spatial pre-splitting budget in extra references per triangle (0 = off).
