@@ -83,6 +83,34 @@ __device__ __forceinline__ bool seg_fetch(const SegIn& in, uint32_t n_seg, uint3
   }
 }
 
+// Like seg_fetch, for up to `want` items (1..64) from one segment.
+__device__ __forceinline__ bool seg_fetch_k(const SegIn& in, uint32_t n_seg, uint32_t home, uint64_t& done,
+                                            uint32_t want, uint32_t& slot0, uint32_t& n_got) {
+  const uint32_t lane = lane_id();
+  for (;;) {
+    uint32_t rem = 0;
+    if (lane < kSegs && !((done >> lane) & 1ull)) {
+      const uint32_t w = __hip_atomic_load(&in.work[lane * kLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      rem = w < n_seg ? n_seg - w : 0u;
+    }
+    const uint64_t m = __ballot(rem > 0u);
+    if (m == 0ull) return false;
+    const uint64_t rot = ((m >> home) | (m << (kSegs - home))) & ((1ull << kSegs) - 1ull);
+    uint32_t sgm = home + static_cast<uint32_t>(__builtin_ctzll(rot));
+    if (sgm >= kSegs) sgm -= kSegs;
+    uint32_t b = 0;
+    if (lane == 0) b = atomicAdd(&in.work[sgm * kLine], want);
+    b = __shfl(b, 0);
+    const uint32_t ns = __shfl(n_seg, static_cast<int>(sgm));
+    if (b < ns) {
+      slot0 = sgm * in.cap + b;
+      n_got = ns - b < want ? ns - b : want;
+      return true;
+    }
+    done |= 1ull << sgm;
+  }
+}
+
 // Per-lane item count of segment `lane` (lanes >= kSegs: 0).
 __device__ __forceinline__ uint32_t seg_counts(const SegIn& in) {
   const uint32_t lane = lane_id();
@@ -125,6 +153,42 @@ struct TraceOut {
   uint32_t nodes, tests;
 };
 
+// Ray/box slab test of the device traversals: the host slab's arithmetic ((plane - o)
+// * inv, each distance within 2^-23 relative of exact, accepted with the relative
+// slack kBoxSlack, so no box the ray touches is culled and traversal equals a
+// brute-force loop), with reciprocals clamped to +-1e30 (zero and tiny components).
+// Every distance is then finite, so min/max can use the hardware v_min/v_max and
+// v_min3/v_max3 instead of compare-and-select pairs (the two agree on non-NaN inputs).
+struct RayBox {
+  V3 o;
+  V3 inv;  // clamped reciprocal direction
+};
+
+__device__ __forceinline__ float clamp_inv(float x) {
+  const float r = 1.f / x;
+  return __builtin_fabsf(r) <= 1e30f ? r : (x < 0.f ? -1e30f : 1e30f);
+}
+
+__device__ __forceinline__ RayBox ray_box_setup(V3 o, V3 d) {
+  RayBox b;
+  b.o = o;
+  b.inv = v3(clamp_inv(d.x), clamp_inv(d.y), clamp_inv(d.z));
+  return b;
+}
+
+__device__ __forceinline__ bool slab_hw(float lox, float hix, float loy, float hiy, float loz, float hiz,
+                                        const RayBox& b, float tmin, float tmax, float& tnear) {
+  const float tx0 = (lox - b.o.x) * b.inv.x, tx1 = (hix - b.o.x) * b.inv.x;
+  const float ty0 = (loy - b.o.y) * b.inv.y, ty1 = (hiy - b.o.y) * b.inv.y;
+  const float tz0 = (loz - b.o.z) * b.inv.z, tz1 = (hiz - b.o.z) * b.inv.z;
+  const float n = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tx0, tx1), __builtin_fminf(ty0, ty1)),
+                                  __builtin_fmaxf(__builtin_fminf(tz0, tz1), tmin));
+  const float f = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)),
+                                  __builtin_fminf(__builtin_fmaxf(tz0, tz1), tmax));
+  tnear = n;
+  return n <= f * kBoxSlack;
+}
+
 __device__ __forceinline__ void load_node(const DevNode2* __restrict__ nodes, int32_t i, float4& r0, float4& r1,
                                           float4& r2, int4& c) {
   const float4* p = reinterpret_cast<const float4*>(nodes + i);
@@ -142,7 +206,7 @@ __device__ __forceinline__ bool traverse_bvh(const DevNode2* __restrict__ nodes,
                                              int32_t root, V3 o, V3 d, float tmin, uint32_t inst, uint32_t* stk,
                                              int sp0, int cap, TraceOut& r, uint32_t* overflow) {
   const RayTri rt = ray_tri_setup(o, d);
-  const V3 inv = safe_inv(d);
+  const RayBox rb = ray_box_setup(o, d);
   int sp = sp0;
   int32_t node = root;
   for (;;) {
@@ -152,8 +216,8 @@ __device__ __forceinline__ bool traverse_bvh(const DevNode2* __restrict__ nodes,
       load_node(nodes, node, r0, r1, r2, c);
       if (STATS) r.nodes++;
       float t0, t1;
-      const bool h0 = slab(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, o, inv, tmin, r.t, t0);
-      const bool h1 = slab(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, o, inv, tmin, r.t, t1);
+      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
       if (h0 && h1) {
         int32_t nearc = c.x, farc = c.y;
         if (t1 < t0) {
@@ -225,7 +289,7 @@ __device__ __forceinline__ bool traverse_ww(const DevNode2* __restrict__ nodes, 
                                             V3 d, float tmin, uint32_t* stk, int cap, TraceOut& r,
                                             uint32_t* overflow) {
   const RayTri rt = ray_tri_setup(o, d);
-  const V3 inv = safe_inv(d);
+  const RayBox rb = ray_box_setup(o, d);
   int sp = 0;
   int32_t node = 0;  // next node: inner index, leaf code, or kTravDone
   int32_t leaf = 0;  // parked leaf code (0 = none)
@@ -242,8 +306,8 @@ __device__ __forceinline__ bool traverse_ww(const DevNode2* __restrict__ nodes, 
       load_node(nodes, node, r0, r1, r2, c);
       if (STATS) r.nodes++;
       float t0, t1;
-      const bool h0 = slab(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, o, inv, tmin, r.t, t0);
-      const bool h1 = slab(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, o, inv, tmin, r.t, t1);
+      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
       if (h0 && h1) {
         int32_t nearc = c.x, farc = c.y;
         if (t1 < t0) {
@@ -307,13 +371,153 @@ __device__ __forceinline__ bool traverse_ww(const DevNode2* __restrict__ nodes, 
   return false;
 }
 
+// Persistent single-level traversal of a segmented ray queue with lane refill
+// ("dynamic fetch", Aila & Laine 2009): the wave runs rounds of the speculative
+// while-while loop above, and before each round lanes whose ray has finished take new
+// rays from the queue once at least kRefill of them are idle, so a few long rays no
+// longer hold the whole wave (48 of 64 measured best on MI355X: a lower threshold
+// refills in small batches and mixes rays from distant queue slots, which costs
+// more cache locality than it saves). load(slot, o, d, tmin, tmax) reads a ray;
+// finish(slot, r, hit) consumes a result (hit: a hit was found).
+constexpr uint32_t kRefill = 48;
+
+template <bool ANY, bool STATS, class Load, class Finish>
+__device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in, uint32_t* stk, uint32_t* overflow,
+                                            TraceOut& stats, Load load, Finish finish) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_seg = seg_counts(in), home = wave_home();
+  const int cap = acc.stack_cap;
+  const DevNode2* __restrict__ nodes = acc.nodes;
+  const float4* __restrict__ tris = acc.tris;
+  uint64_t seg_done = 0;
+  bool drained = false;
+  uint32_t slot = kNoHit;
+  V3 o = v3(0.f), d = v3(0.f);
+  RayBox rb{};
+  float tmin = 0.f;
+  RayTri rt{};
+  TraceOut r;
+  r.t = 0.f;
+  r.b1 = r.b2 = 0.f;
+  r.prim = kNoHit;
+  r.inst = kNoInstance;
+  int sp = 0;
+  int32_t node = kTravDone, leaf = 0;
+  auto pop = [&]() -> int32_t {
+    if (sp == 0) return kTravDone;
+    --sp;
+    return static_cast<int32_t>(stk[sp * 64]);
+  };
+  for (;;) {
+    const uint64_t idle = __ballot(slot == kNoHit);
+    const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+    if (!drained && n_idle >= kRefill) {
+      uint32_t base = 0, got = 0;
+      if (seg_fetch_k(in, n_seg, home, seg_done, n_idle, base, got)) {
+        const uint32_t rank = prefix_count(idle);
+        if (slot == kNoHit && rank < got) {
+          slot = base + rank;
+          float tmax;
+          load(slot, o, d, tmin, tmax);
+          rt = ray_tri_setup(o, d);
+          rb = ray_box_setup(o, d);
+          r.t = tmax;
+          r.b1 = r.b2 = 0.f;
+          r.prim = kNoHit;
+          r.inst = kNoInstance;
+          sp = 0;
+          node = 0;
+          leaf = 0;
+        }
+      } else {
+        drained = true;
+      }
+    }
+    if (!__any(slot != kNoHit)) break;  // drained and every lane idle
+    if (slot == kNoHit) continue;
+    bool hit = false;
+    // inner nodes
+    while (node >= 0) {
+      float4 r0, r1, r2;
+      int4 c;
+      load_node(nodes, node, r0, r1, r2, c);
+      if (STATS) stats.nodes++;
+      float t0, t1;
+      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+      if (h0 && h1) {
+        int32_t nearc = c.x, farc = c.y;
+        if (t1 < t0) {
+          nearc = c.y;
+          farc = c.x;
+        }
+        if (sp < cap) {
+          stk[sp * 64] = static_cast<uint32_t>(farc);
+          ++sp;
+        } else {
+          atomicOr(overflow, 1u);
+        }
+        node = nearc;
+      } else if (h0) {
+        node = c.x;
+      } else if (h1) {
+        node = c.y;
+      } else {
+        node = pop();
+      }
+      if (node < 0 && node != kTravDone && leaf == 0) {
+        leaf = node;
+        node = pop();
+      }
+      if (__all(leaf != 0)) break;
+    }
+    // leaves
+    while (leaf != 0) {
+      const uint32_t code = ~static_cast<uint32_t>(leaf);
+      const uint32_t first = code >> 3, cnt = code & 7u;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const float4* tp = tris + static_cast<size_t>(first + k) * 3u;
+        const float4 a = tp[0], b = tp[1], cc = tp[2];
+        if (STATS) stats.tests++;
+        TriHit th;
+        if (intersect_triangle(rt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), tmin, r.t, th)) {
+          const uint32_t prim = __float_as_uint(a.w);
+          if (th.t < r.t || prim < r.prim) {
+            r.t = th.t;
+            r.b1 = th.b1;
+            r.b2 = th.b2;
+            r.prim = prim;
+          }
+          if (ANY) {
+            hit = true;
+            break;
+          }
+        }
+      }
+      if (ANY && hit) break;
+      if (node < 0 && node != kTravDone) {
+        leaf = node;
+        node = pop();
+      } else {
+        leaf = 0;
+      }
+    }
+    if (hit || (node == kTravDone && leaf == 0)) {
+      finish(slot, r, hit || r.prim != kNoHit);
+      slot = kNoHit;
+      node = kTravDone;
+      leaf = 0;
+    }
+  }
+}
+
 // Full trace (single or two-level). r.t must hold tmax on entry.
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trace_ray(const DevAccel& acc, V3 o, V3 d, float tmin, uint32_t* stk,
                                           TraceOut& r, uint32_t* overflow) {
   if (!acc.two_level) return traverse_ww<ANY, STATS>(acc.nodes, acc.tris, o, d, tmin, stk, acc.stack_cap, r, overflow);
   // top level: leaf ranges index the instance array in TLAS build order
-  const V3 inv = safe_inv(d);
+  const RayBox rb = ray_box_setup(o, d);
   int sp = 0;
   int32_t node = 0;
   const int cap = acc.stack_cap;
@@ -324,8 +528,8 @@ __device__ __forceinline__ bool trace_ray(const DevAccel& acc, V3 o, V3 d, float
       load_node(acc.tlas, node, r0, r1, r2, c);
       if (STATS) r.nodes++;
       float t0, t1;
-      const bool h0 = slab(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, o, inv, tmin, r.t, t0);
-      const bool h1 = slab(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, o, inv, tmin, r.t, t1);
+      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
       if (h0 && h1) {
         int32_t nearc = c.x, farc = c.y;
         if (t1 < t0) {
@@ -452,6 +656,68 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevAccel acc, RayQueue q, Hit
       wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 0]), r.nodes);
       wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 2]), r.tests);
     }
+  }
+}
+
+// Single-level scenes: Extend and Shadow through trace_queue.
+template <bool STATS>
+__global__ __launch_bounds__(kBlock) void k_extend_sl(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane_id();
+  TraceOut st;
+  st.nodes = 0;
+  st.tests = 0;
+  trace_queue<false, STATS>(
+      acc, in, stk, &ctr[kCtrOverflow], st,
+      [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
+        const float4 a = q.o_tmin[slot];
+        const float4 b = q.d_tmax[slot];
+        o = v3(a.x, a.y, a.z);
+        d = v3(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+      },
+      [&](uint32_t slot, const TraceOut& r, bool) {
+        h.thit[slot] = make_float4(r.t, r.b1, r.b2, __uint_as_float(r.prim));
+      });
+  if (STATS) {
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 0]), st.nodes);
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 2]), st.tests);
+  }
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(kBlock) void k_shadow_sl(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
+                                                       uint32_t* ctr) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane_id();
+  TraceOut st;
+  st.nodes = 0;
+  st.tests = 0;
+  trace_queue<true, STATS>(
+      acc, in, stk, &ctr[kCtrOverflow], st,
+      [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
+        const float4 a = sq.o_tmin[slot];
+        const float4 b = sq.d_tmax[slot];
+        o = v3(a.x, a.y, a.z);
+        d = v3(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+      },
+      [&](uint32_t slot, const TraceOut&, bool occluded) {
+        if (!occluded) {
+          const float4 c = sq.contrib[slot];
+          const uint32_t path = __float_as_uint(c.w);
+          float4 L = radiance[path];
+          L.x += c.x;
+          L.y += c.y;
+          L.z += c.z;
+          radiance[path] = L;
+        }
+      });
+  if (STATS) {
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 4]), st.nodes);
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 6]), st.tests);
   }
 }
 
@@ -854,10 +1120,17 @@ size_t stack_lds_bytes(const DevAccel& acc) { return static_cast<size_t>(acc.sta
 
 int trace_blocks_per_cu(const DevAccel& acc) {
   int a = 0, b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false>, kBlock, stack_lds_bytes(acc)) != hipSuccess)
-    a = 4;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kBlock, stack_lds_bytes(acc)) != hipSuccess)
-    b = 4;
+  const size_t lds = stack_lds_bytes(acc);
+  hipError_t ea, eb;
+  if (acc.two_level) {
+    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false>, kBlock, lds);
+    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kBlock, lds);
+  } else {
+    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, lds);
+    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, lds);
+  }
+  if (ea != hipSuccess) a = 4;
+  if (eb != hipSuccess) b = 4;
   const int m = a < b ? a : b;
   return m > 0 ? m : 1;
 }
@@ -879,6 +1152,14 @@ void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQu
 
 void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr, int grid, bool stats,
                    hipStream_t stream) {
+  if (!acc.two_level) {
+    if (stats)
+      hipLaunchKernelGGL(k_extend_sl<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in, ctr);
+    else
+      hipLaunchKernelGGL(k_extend_sl<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in,
+                         ctr);
+    return;
+  }
   if (stats)
     hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in, ctr);
   else
@@ -894,6 +1175,15 @@ void launch_shade(const SceneView& sv, const Material* sampling_mats, const Rend
 
 void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,
                    bool stats, hipStream_t stream) {
+  if (!acc.two_level) {
+    if (stats)
+      hipLaunchKernelGGL(k_shadow_sl<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance,
+                         in, ctr);
+    else
+      hipLaunchKernelGGL(k_shadow_sl<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance,
+                         in, ctr);
+    return;
+  }
   if (stats)
     hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance, in,
                        ctr);
