@@ -59,23 +59,24 @@ def make_scene(name):
     return scenes.instanced(), "instanced-1000x20480-tris 3840x2160 4spp depth6"
 
 
-def cpu_baseline(scene, threads):
+def cpu_baseline(scene, threads, spp=2):
     """Oracle (host build of the same per-vertex math) on a bounded sample of the same
-    workload: the same camera at 256x144, 1 spp, same depth."""
+    workload: the same camera and resolution at `spp` samples per pixel (8 in the GPU
+    frame), same depth, about 10 s of host work on 16 threads."""
     from oracle import Oracle
     t0 = time.perf_counter()
     orc = Oracle(scene)
     build_s = time.perf_counter() - t0
     r = dict(scene["render"])
-    W, H = 256, max(1, int(round(256 * r["height"] / r["width"])))
+    W, H = r["width"], r["height"]
     t0 = time.perf_counter()
-    out = orc.render(render=dict(width=W, height=H, spp=1), threads=threads)
+    out = orc.render(render=dict(width=W, height=H, spp=spp), threads=threads)
     dt = time.perf_counter() - t0
     rays = int(out["counts"].sum())
     orc.close()
     return {
         "value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-        "sample": f"same scene and camera at {W}x{H}, 1 spp, depth {r['max_depth']}: {rays} rays in {dt:.2f} s "
+        "sample": f"same scene and camera at {W}x{H}, {spp} spp, depth {r['max_depth']}: {rays} rays in {dt:.2f} s "
                   f"(oracle BVH build {build_s:.1f} s not timed)",
     }
 
@@ -152,15 +153,18 @@ def main():
     ext_bytes = bytes_per_ext * ext_rays
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
     traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", "k_extend_soup1m_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "k_extend_sl_soup1m_traffic.json")
     if args.workload == "soup1m" and os.path.exists(tpath):
         with open(tpath) as f:
             traffic = round(json.load(f)["traffic_bytes_per_launch"])
-        traffic_src = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/k_extend_soup1m_hbm_traffic.md)"
+        traffic_src = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/k_extend_sl_soup1m_hbm_traffic.md)"
+    avg_launch_s = ext_ms / max(1, launches) * 1e-3
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-        "kernel": "k_extend", "avg_launch_ms": round(ext_ms / max(1, launches), 4),
+        "kernel": "k_extend" if info["two_level"] else "k_extend_sl<false>",
+        "avg_launch_ms": round(ext_ms / max(1, launches), 4),
+        "dram_gbs": round(traffic / avg_launch_s / 1e9, 1) if traffic else None,
         "bytes_per_ray": round(bytes_per_ext, 1), "nodes_per_ray": round(tc["closest_nodes_per_ray"], 2),
         "tris_per_ray": round(tc["closest_tris_per_ray"], 2),
     }
