@@ -157,6 +157,55 @@ trt_status trt_set_instances(trt_context* ctx, const trt_instance* inst, uint32_
 trt_status trt_build(trt_context* ctx, const trt_build_opts* opts);
 trt_status trt_bvh_info_get(trt_context* ctx, trt_bvh_info* info);
 
+/* Scene files (replaces SceneFile::Load, Tracer/FileIO/SceneFile.cpp:683-731, and
+ * ModelFile::Import, Tracer/FileIO/ModelFile.cpp:331-448, for Wavefront OBJ/MTL models):
+ * the reference's JSON scene format with its "models", "instances" (transform
+ * operations matrix / translate / rotate-x|y|z / scale), "materials", "camera",
+ * "renderer" and "sky" sections. Instance matrices are composed like the reference and
+ * read row-major, as its traversal reads them (Tracer/Resources/Instance.cpp:39-49).
+ * Materials become Disney materials (model 0) with emission = emissive x diffuse, the
+ * radiance the reference shades with (Tracer/CUDA/GPU/CudaMaterial.h:158). Texture maps
+ * are not supported: each one is reported as a warning. */
+typedef struct trt_scene_file_info {
+  uint32_t n_models;      /* models used by instances */
+  uint32_t n_instances;
+  uint32_t n_materials;   /* all models' materials, concatenated in model order */
+  uint32_t n_triangles;   /* all models' triangles (each model counted once) */
+  uint32_t has_camera;
+  trt_camera camera;      /* fov converted to radians */
+  uint32_t spp;           /* renderer.multisample */
+  uint32_t max_depth;     /* renderer.maxdepth */
+  float ray_epsilon;      /* renderer.ray-epsilon */
+  float ao_dist;          /* renderer.aodist */
+  float zdepth_max;       /* renderer.zdepthmax */
+  uint32_t sky_enabled;
+  uint32_t sky_draw_sun;
+  float sky_dir[3];
+  float sky_angular_diameter;  /* degrees (the reference's unit) */
+  float sky_intensity;
+  float sky_turbidity;
+  float sky_bias;
+  uint32_t n_warnings;
+} trt_scene_file_info;
+
+/* Parses a scene file and its OBJ models on the host (no device use) and copies the
+ * result out. Pass NULL arrays to query the sizes in *info first. Arrays: positions and
+ * normals (9 floats per triangle, models concatenated, object space), mat_ids (model-
+ * local), model_tri_offset / model_tri_count / model_mat_offset (one per model),
+ * materials (n_materials), instances (n_instances; mesh = model index, material_offset =
+ * its model's first material). `msg` (msg_len bytes, may be NULL) receives the error,
+ * or on success the warnings, one per line. */
+trt_status trt_scene_file_read(const char* path, trt_scene_file_info* info, float* positions, float* normals,
+                               uint32_t* mat_ids, uint32_t* model_tri_offset, uint32_t* model_tri_count,
+                               uint32_t* model_mat_offset, trt_material* materials, trt_instance* instances,
+                               char* msg, size_t msg_len);
+
+/* Loads a scene file into the context: replaces the scene (meshes, materials,
+ * instances), builds it with `opts` (NULL = defaults) and sets the file's camera. The
+ * file's render settings come back in *info for the caller's trt_render_opts. */
+trt_status trt_load_scene_file(trt_context* ctx, const char* path, const trt_build_opts* opts,
+                               trt_scene_file_info* info);
+
 /* Camera and frame (replaces Renderer::SetCamera / RenderFrame / Reset,
  * Tracer/Renderer/Renderer.h:45-61; DeviceRenderer::RenderFrame, Tracer/Renderer/DeviceRenderer.cpp:97-133). */
 trt_status trt_set_camera(trt_context* ctx, const trt_camera* cam);

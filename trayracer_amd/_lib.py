@@ -37,6 +37,31 @@ class TrtCamera(ctypes.Structure):
     ]
 
 
+class TrtSceneFileInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_models", ctypes.c_uint32), ("n_instances", ctypes.c_uint32), ("n_materials", ctypes.c_uint32),
+        ("n_triangles", ctypes.c_uint32), ("has_camera", ctypes.c_uint32), ("camera", TrtCamera),
+        ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("ray_epsilon", ctypes.c_float),
+        ("ao_dist", ctypes.c_float), ("zdepth_max", ctypes.c_float),
+        ("sky_enabled", ctypes.c_uint32), ("sky_draw_sun", ctypes.c_uint32), ("sky_dir", ctypes.c_float * 3),
+        ("sky_angular_diameter", ctypes.c_float), ("sky_intensity", ctypes.c_float),
+        ("sky_turbidity", ctypes.c_float), ("sky_bias", ctypes.c_float), ("n_warnings", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            if k == "camera":
+                v = dict(position=tuple(v.position), target=tuple(v.target), up=tuple(v.up), fov=v.fov,
+                         aperture=v.aperture, focal_dist=v.focal_dist, distortion=v.distortion,
+                         bokeh_sides=v.bokeh_sides, bokeh_rotation=v.bokeh_rotation)
+            elif k == "sky_dir":
+                v = tuple(v)
+            out[k] = v
+        return out
+
+
 class TrtBuildOpts(ctypes.Structure):
     _fields_ = [("bins", ctypes.c_int32), ("max_leaf", ctypes.c_int32), ("threads", ctypes.c_int32),
                 ("presplit", ctypes.c_float)]
@@ -114,6 +139,10 @@ PROTOTYPES = [
     ("trt_set_instances", S, [P, ctypes.POINTER(TrtInstance), ctypes.c_uint32]),
     ("trt_build", S, [P, ctypes.POINTER(TrtBuildOpts)]),
     ("trt_bvh_info_get", S, [P, ctypes.POINTER(TrtBvhInfo)]),
+    ("trt_scene_file_read", S, [ctypes.c_char_p, ctypes.POINTER(TrtSceneFileInfo), c_float_p, c_float_p, c_u32_p,
+                                c_u32_p, c_u32_p, c_u32_p, ctypes.POINTER(TrtMaterial), ctypes.POINTER(TrtInstance),
+                                ctypes.c_char_p, ctypes.c_size_t]),
+    ("trt_load_scene_file", S, [P, ctypes.c_char_p, ctypes.POINTER(TrtBuildOpts), ctypes.POINTER(TrtSceneFileInfo)]),
     ("trt_set_camera", S, [P, ctypes.POINTER(TrtCamera)]),
     ("trt_render", S, [P, ctypes.POINTER(TrtRenderOpts), ctypes.POINTER(TrtStats)]),
     ("trt_reset", S, [P]),

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/trt.h"
+#include "api_internal.h"
 #include "bvh_build.h"
 #include "denoiser.h"
 #include "device.h"
@@ -418,6 +419,8 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
 }
 
 }  // namespace
+
+trt_status trt_internal_fail(trt_context* ctx, trt_status s, const std::string& msg) { return fail(ctx, s, msg); }
 
 extern "C" {
 

@@ -6,6 +6,7 @@ Statistics, SampleCount. Device work runs in libtrt.so on the GPU; nothing here
 computes pixels.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -116,6 +117,18 @@ class Renderer:
             self.SetCamera(scene["camera"])
         if "render" in scene:
             self.SetSettings(KernelSettings.from_dict(scene["render"]))
+
+    def LoadScene(self, path, bins=32, max_leaf=4, threads=0, presplit=3.0):
+        """Loads a scene file in the reference's JSON format (OBJ models) and builds it
+        (SceneFile::Load, Tracer/FileIO/SceneFile.cpp:683-731). The file's render settings
+        (multisample, maxdepth, ray-epsilon) replace the current settings; the film size,
+        seed and Russian-roulette depth are kept. Returns the file's info as a dict."""
+        info = _lib.TrtSceneFileInfo()
+        bo = _lib.TrtBuildOpts(bins, max_leaf, threads, presplit)
+        self._check(self.lib.trt_load_scene_file(self.ctx, os.fsencode(path), ctypes.byref(bo), ctypes.byref(info)))
+        s = self.settings
+        s.spp, s.max_depth, s.ray_epsilon = info.spp, info.max_depth, info.ray_epsilon
+        return info.as_dict()
 
     def bvh_info(self):
         info = _lib.TrtBvhInfo()

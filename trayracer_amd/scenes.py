@@ -262,3 +262,48 @@ def instanced(n_instances=1000, subdiv=5, seed=2, width=3840, height=2160, spp=4
         render=dict(width=width, height=height, spp=spp, max_depth=max_depth, rr_depth=rr_depth, seed=seed,
                     ray_epsilon=1e-4, clamp=0.0),
     )
+
+
+def from_file(path, width=256, height=256, rr_depth=3, seed=0):
+    """Reads a scene file in the reference's JSON format (and its OBJ models) on the
+    host, through the library's parser (trt_scene_file_read; no device use), into the
+    scene dict used by Renderer.UpdateScene and the oracle. Returns (scene, info,
+    warnings): the file's camera and render settings (multisample -> spp, maxdepth,
+    ray-epsilon) go into the dict; film size, seed and RR depth are the arguments."""
+    import ctypes
+    import os
+
+    from . import _lib
+    lib = _lib.load()
+    info = _lib.TrtSceneFileInfo()
+    msg = ctypes.create_string_buffer(1 << 16)
+    p = os.fsencode(path)
+    st = lib.trt_scene_file_read(p, ctypes.byref(info), None, None, None, None, None, None, None, None, msg, len(msg))
+    if st != 0:
+        raise _lib.TrtError(st, msg.value.decode(errors="replace"))
+    T, M, K, I = info.n_triangles, info.n_models, info.n_materials, info.n_instances
+    pos = np.zeros((T, 9), np.float32)
+    nrm = np.zeros((T, 9), np.float32)
+    mids = np.zeros(T, np.uint32)
+    toff = np.zeros(M, np.uint32)
+    tcnt = np.zeros(M, np.uint32)
+    moff = np.zeros(M, np.uint32)
+    mats = (_lib.TrtMaterial * max(K, 1))()
+    insts = (_lib.TrtInstance * max(I, 1))()
+    fp = lambda a: a.ctypes.data_as(_lib.c_float_p)  # noqa: E731
+    up = lambda a: a.ctypes.data_as(_lib.c_u32_p)  # noqa: E731
+    st = lib.trt_scene_file_read(p, ctypes.byref(info), fp(pos), fp(nrm), up(mids), up(toff), up(tcnt), up(moff),
+                                 mats, insts, msg, len(msg))
+    if st != 0:
+        raise _lib.TrtError(st, msg.value.decode(errors="replace"))
+    mat_arr = np.frombuffer(bytes(mats), np.float32).reshape(-1, 17)[:K].copy()
+    inst_arr = np.frombuffer(bytes(insts), np.float32).reshape(-1, 14)[:I].copy()
+    d = info.as_dict()
+    cam = d["camera"] if info.has_camera else dict(position=(0.0, 0.0, -1.0), target=(0.0, 0.0, 0.0),
+                                                      up=(0.0, 1.0, 0.0), fov=math.radians(90.0))
+    scene = dict(positions=pos, normals=nrm, mat_ids=mids, meshes=[(int(o), int(c)) for o, c in zip(toff, tcnt)],
+                 materials=mat_arr, instances=inst_arr, camera=cam,
+                 render=dict(width=width, height=height, spp=int(info.spp), max_depth=int(info.max_depth),
+                             rr_depth=rr_depth, seed=seed, ray_epsilon=float(info.ray_epsilon), clamp=0.0))
+    warnings = [w for w in msg.value.decode(errors="replace").split("\n") if w]
+    return scene, d, warnings
