@@ -206,6 +206,21 @@ trt_status trt_scene_file_read(const char* path, trt_scene_file_info* info, floa
 trt_status trt_load_scene_file(trt_context* ctx, const char* path, const trt_build_opts* opts,
                                trt_scene_file_info* info);
 
+/* Sky seen by rays that leave the scene (replaces Sky and SampleSky,
+ * Tracer/Renderer/Sky.h:10-66, Tracer/CUDA/GPU/CudaSky.h:44-156): the reference's
+ * Preetham-style daylight model. Not sampled by next-event estimation (the reference
+ * compiles sky NEE out, Tracer/CUDA/GPU/CudaLights.h:11); the sun disc is drawn for camera
+ * rays only. Resets accumulation. Disabled by default (trt_clear_scene disables it). */
+typedef struct trt_sky {
+  uint32_t enabled;
+  uint32_t draw_sun;
+  float sun_dir[3];          /* towards the sun (normalised here) */
+  float angular_diameter;    /* sun disc, arc minutes (32) */
+  float intensity;           /* sun intensity (1000) */
+  float turbidity;           /* atmosphere turbidity (4) */
+} trt_sky;
+trt_status trt_set_sky(trt_context* ctx, const trt_sky* sky);
+
 /* Camera and frame (replaces Renderer::SetCamera / RenderFrame / Reset,
  * Tracer/Renderer/Renderer.h:45-61; DeviceRenderer::RenderFrame, Tracer/Renderer/DeviceRenderer.cpp:97-133). */
 trt_status trt_set_camera(trt_context* ctx, const trt_camera* cam);

@@ -395,6 +395,24 @@ void orc_set_material_params(void* p, const float* params, uint32_t n, uint32_t 
   }
 }
 
+// Sky seen by rays that leave the scene (same parameters as trt_set_sky).
+void orc_set_sky(void* p, uint32_t enabled, uint32_t draw_sun, const float* dir, float angular_diameter,
+                 float intensity, float turbidity) {
+  OScene* s = static_cast<OScene*>(p);
+  s->view.sky = make_sky(enabled != 0, draw_sun != 0, dir, angular_diameter, intensity, turbidity);
+}
+
+// Sky radiance along n unit directions (3 floats each) -> out (3 floats each).
+void orc_sky_radiance(void* p, const float* dirs, uint32_t n, uint32_t sun, float* out) {
+  OScene* s = static_cast<OScene*>(p);
+  for (uint32_t i = 0; i < n; ++i) {
+    const V3 r = sky_radiance(s->view.sky, v3(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]), sun != 0);
+    out[i * 3] = r.x;
+    out[i * 3 + 1] = r.y;
+    out[i * 3 + 2] = r.z;
+  }
+}
+
 void orc_scene_destroy(void* p) { delete static_cast<OScene*>(p); }
 
 uint32_t orc_light_count(void* p) { return uint32_t(static_cast<OScene*>(p)->lt.lights.size()); }

@@ -25,6 +25,8 @@ _PROTO = [
     ("orc_trace_any", None, [_P, _f, ctypes.c_uint32, _u8, ctypes.c_int, ctypes.c_int]),
     ("orc_render", None, [_P, _f, _u, _f, _f, _f, _d, _f, _u64, ctypes.c_int]),
     ("orc_set_material_params", None, [_P, _f, ctypes.c_uint32, ctypes.c_uint32]),
+    ("orc_set_sky", None, [_P, ctypes.c_uint32, ctypes.c_uint32, _f, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("orc_sky_radiance", None, [_P, _f, ctypes.c_uint32, ctypes.c_uint32, _f]),
     ("orc_backward", None, [_P, _f, _u, _f, _f, _d, ctypes.c_int]),
     ("orc_pcg32", None, [ctypes.c_uint64, ctypes.c_uint64, _u, ctypes.c_uint32]),
     ("orc_rng_floats", None, [ctypes.c_uint32] * 4 + [_f, ctypes.c_uint32]),
@@ -103,6 +105,21 @@ class Oracle:
         self.handle = self.lib.orc_scene_create(
             fp(pos), None if nrm is None else fp(nrm), up(mid), pos.shape[0], up(moff), up(mcnt), len(meshes),
             fp(mats), mats.shape[0], None if inst is None else fp(inst), ni)
+        if scene.get("sky") is not None:
+            self.set_sky(scene["sky"])
+
+    def set_sky(self, sky):
+        """sky: dict(enabled, draw_sun, dir, angular_diameter (arc minutes), intensity, turbidity)."""
+        d = np.ascontiguousarray(sky.get("dir", (0.0, 1.0, 0.0)), np.float32)
+        self.lib.orc_set_sky(self.handle, int(sky.get("enabled", True)), int(sky.get("draw_sun", True)), fp(d),
+                             float(sky.get("angular_diameter", 32.0)), float(sky.get("intensity", 1000.0)),
+                             float(sky.get("turbidity", 4.0)))
+
+    def sky_radiance(self, dirs, sun=True):
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.empty_like(dirs)
+        self.lib.orc_sky_radiance(self.handle, fp(dirs), dirs.shape[0], int(sun), fp(out))
+        return out
 
     def close(self):
         if self.handle:

@@ -16,8 +16,14 @@ def rend():
     r.close()
 
 
-def _setup(rend, width, height, spp, seed=3):
+SKY = dict(enabled=True, draw_sun=True, dir=(0.3, 0.8, -0.5), angular_diameter=32.0, intensity=1000.0,
+           turbidity=4.0)
+
+
+def _setup(rend, width, height, spp, seed=3, sky=None):
     sc, target = scenes.cornell_diff(width=width, height=height, spp=spp, max_depth=4, seed=seed)
+    if sky is not None:
+        sc["sky"] = sky
     rend.UpdateScene(sc)
     theta = scenes.material_params(sc["materials"])
     rend.set_material_params(target, which=3)
@@ -39,9 +45,10 @@ def _significant(g, frac=1e-3):
     return np.abs(g) > frac * np.abs(g).max()
 
 
-def test_backward_matches_oracle_fp64(rend):
+@pytest.mark.parametrize("sky", [None, SKY], ids=["emitters", "emitters+sky"])
+def test_backward_matches_oracle_fp64(rend, sky):
     from oracle import Oracle
-    sc, theta, target_img, img, dldi = _setup(rend, 64, 64, 16)
+    sc, theta, target_img, img, dldi = _setup(rend, 64, 64, 16, sky=sky)
     g = rend.backward(dldi)
     orc = Oracle(sc)
     orc.set_material_params(theta, which=3)

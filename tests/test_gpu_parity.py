@@ -155,3 +155,18 @@ def test_pick_ray(rend):
     p = rend.PickRay(32, 32)
     assert p["primitive"] != 0xFFFFFFFF
     assert abs(p["direction"][2] - 1.0) < 0.01
+
+
+def test_path_radiance_with_sky(rend):
+    """Open Cornell box under the daylight sky: rays leaving through the open side and
+    camera rays that miss pick up sky radiance (sun disc for camera rays)."""
+    sc = scenes.cornell(width=48, height=40, spp=2, max_depth=5, seed=11)
+    sc["camera"] = dict(sc["camera"], position=(0.5, 0.6, -2.2), fov=1.2)  # box plus surrounding sky
+    sc["sky"] = dict(enabled=True, draw_sun=True, dir=(0.2, 0.3, -1.0), angular_diameter=32.0, intensity=1000.0,
+                     turbidity=4.0)
+    rend.UpdateScene(sc)
+    rend.RenderFrame()
+    ref = _oracle(sc).render()
+    _compare_paths(rend.path_radiance(), ref["path_rgb"], "sky")
+    np.testing.assert_array_equal(rend.albedo()[..., :3], ref["aov"][..., 0:3])
+    assert (ref["aov"][..., 0:3] > 0).all()  # every first hit or miss has an albedo (sky for misses)

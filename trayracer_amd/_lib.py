@@ -62,6 +62,11 @@ class TrtSceneFileInfo(ctypes.Structure):
         return out
 
 
+class TrtSky(ctypes.Structure):
+    _fields_ = [("enabled", ctypes.c_uint32), ("draw_sun", ctypes.c_uint32), ("sun_dir", ctypes.c_float * 3),
+                ("angular_diameter", ctypes.c_float), ("intensity", ctypes.c_float), ("turbidity", ctypes.c_float)]
+
+
 class TrtBuildOpts(ctypes.Structure):
     _fields_ = [("bins", ctypes.c_int32), ("max_leaf", ctypes.c_int32), ("threads", ctypes.c_int32),
                 ("presplit", ctypes.c_float)]
@@ -144,6 +149,7 @@ PROTOTYPES = [
                                 ctypes.c_char_p, ctypes.c_size_t]),
     ("trt_load_scene_file", S, [P, ctypes.c_char_p, ctypes.POINTER(TrtBuildOpts), ctypes.POINTER(TrtSceneFileInfo)]),
     ("trt_set_camera", S, [P, ctypes.POINTER(TrtCamera)]),
+    ("trt_set_sky", S, [P, ctypes.POINTER(TrtSky)]),
     ("trt_render", S, [P, ctypes.POINTER(TrtRenderOpts), ctypes.POINTER(TrtStats)]),
     ("trt_reset", S, [P]),
     ("trt_sample_count", ctypes.c_uint32, [P]),

@@ -91,6 +91,7 @@ struct trt_context {
   bool two_level = false;
   int stack_cap = 1;
   trt_bvh_info info{};
+  SkyParams sky{};  // disabled
 
   // camera / film
   bool camera_set = false;
@@ -300,6 +301,7 @@ SceneView view_of(const trt_context* ctx) {
   s.total_power = ctx->total_power;
   s.n_tris = uint32_t(ctx->mat_ids.size());
   s.n_materials = uint32_t(ctx->materials.size());
+  s.sky = ctx->sky;
   return s;
 }
 
@@ -475,8 +477,20 @@ trt_status trt_clear_scene(trt_context* ctx) {
   ctx->mat_ids.clear();
   ctx->meshes.clear();
   ctx->instances_in.clear();
+  ctx->sky = SkyParams{};
   ctx->built = false;
   ctx->dirty = true;
+  return TRT_OK;
+}
+
+trt_status trt_set_sky(trt_context* ctx, const trt_sky* sky) {
+  if (!ctx || !sky) return TRT_INVALID_ARG;
+  if (!finite_all(sky->sun_dir, 3) || !(sky->angular_diameter >= 0.f) || !(sky->angular_diameter < 10800.f) ||
+      !(sky->intensity >= 0.f) || !(sky->intensity < 1e30f) || !(sky->turbidity >= 0.f) || !(sky->turbidity < 1e6f))
+    return fail(ctx, TRT_INVALID_ARG, "invalid sky parameters");
+  ctx->sky = make_sky(sky->enabled != 0, sky->draw_sun != 0, sky->sun_dir, sky->angular_diameter, sky->intensity,
+                      sky->turbidity);
+  ctx->sample_count = 0;
   return TRT_OK;
 }
 

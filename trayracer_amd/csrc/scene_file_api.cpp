@@ -156,6 +156,14 @@ trt_status trt_load_scene_file(trt_context* ctx, const char* path, const trt_bui
   if ((s = trt_set_instances(ctx, inst.data(), uint32_t(inst.size()))) != TRT_OK) return s;
   if ((s = trt_build(ctx, opts)) != TRT_OK) return s;
   if (sf.has_camera && (s = trt_set_camera(ctx, &local.camera)) != TRT_OK) return s;
+  trt_sky sky;
+  sky.enabled = local.sky_enabled;
+  sky.draw_sun = local.sky_draw_sun;
+  for (int k = 0; k < 3; ++k) sky.sun_dir[k] = local.sky_dir[k];
+  sky.angular_diameter = local.sky_angular_diameter;
+  sky.intensity = local.sky_intensity;
+  sky.turbidity = local.sky_turbidity;
+  if ((s = trt_set_sky(ctx, &sky)) != TRT_OK) return s;
   if (info) *info = local;
   return TRT_OK;
 }

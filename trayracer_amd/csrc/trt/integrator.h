@@ -117,7 +117,15 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
   out.normal = v3(0.f);
   out.vdepth = 0.f;
   if (ADJ) adj->mat = kNoHit;
-  if (h.prim == kNoHit) return;  // no sky: the configured scenes are emitter-lit
+  if (h.prim == kNoHit) {  // the ray left the scene: sky radiance (PathTracing.h:40-55)
+    if (s.sky.enabled) {
+      const bool sun = s.sky.draw_sun != 0u && (depth == 0 || (ps.flags & kFlagSpecular));
+      const V3 sky = sky_radiance(s.sky, ps.d, sun);
+      out.L = clamp_contrib(ps.T * sky, rp.clamp);
+      if (depth == 0) out.albedo = sky;
+    }
+    return;
+  }
 
   const SurfacePoint sp = surface_at(s, h, ps.o, ps.d);
   const Material& m = s.materials[sp.mat];

@@ -15,6 +15,7 @@
 #include "common.h"
 #include "intersect.h"
 #include "sampling.h"
+#include "sky.h"
 
 namespace trt {
 
@@ -97,6 +98,7 @@ struct SceneView {
   float total_power;
   uint32_t n_tris;
   uint32_t n_materials;
+  SkyParams sky;                // radiance of rays that leave the scene (disabled: black)
 };
 
 struct Hit {

@@ -113,6 +113,8 @@ class Renderer:
             self._check(lib.trt_set_instances(ctx, ci, len(inst)))
         bo = _lib.TrtBuildOpts(bins, max_leaf, threads, presplit)
         self._check(lib.trt_build(ctx, ctypes.byref(bo)))
+        if scene.get("sky") is not None:
+            self.SetSky(scene["sky"])
         if "camera" in scene:
             self.SetCamera(scene["camera"])
         if "render" in scene:
@@ -134,6 +136,14 @@ class Renderer:
         info = _lib.TrtBvhInfo()
         self._check(self.lib.trt_bvh_info_get(self.ctx, ctypes.byref(info)))
         return info.as_dict()
+
+    def SetSky(self, sky):
+        """sky: dict(enabled, draw_sun, dir, angular_diameter (arc minutes), intensity,
+        turbidity) as in the reference's scene files (Tracer/Renderer/Sky.h)."""
+        c = _lib.TrtSky(int(sky.get("enabled", True)), int(sky.get("draw_sun", True)),
+                        (ctypes.c_float * 3)(*sky.get("dir", (0.0, 1.0, 0.0))), float(sky.get("angular_diameter", 32.0)),
+                        float(sky.get("intensity", 1000.0)), float(sky.get("turbidity", 4.0)))
+        self._check(self.lib.trt_set_sky(self.ctx, ctypes.byref(c)))
 
     # -- camera / settings -----------------------------------------------------------
     def SetCamera(self, cam):

@@ -301,8 +301,11 @@ def from_file(path, width=256, height=256, rr_depth=3, seed=0):
     d = info.as_dict()
     cam = d["camera"] if info.has_camera else dict(position=(0.0, 0.0, -1.0), target=(0.0, 0.0, 0.0),
                                                       up=(0.0, 1.0, 0.0), fov=math.radians(90.0))
+    sky = dict(enabled=bool(info.sky_enabled), draw_sun=bool(info.sky_draw_sun), dir=tuple(info.sky_dir),
+               angular_diameter=float(info.sky_angular_diameter), intensity=float(info.sky_intensity),
+               turbidity=float(info.sky_turbidity))
     scene = dict(positions=pos, normals=nrm, mat_ids=mids, meshes=[(int(o), int(c)) for o, c in zip(toff, tcnt)],
-                 materials=mat_arr, instances=inst_arr, camera=cam,
+                 materials=mat_arr, instances=inst_arr, camera=cam, sky=sky,
                  render=dict(width=width, height=height, spp=int(info.spp), max_depth=int(info.max_depth),
                              rr_depth=rr_depth, seed=seed, ray_epsilon=float(info.ray_epsilon), clamp=0.0))
     warnings = [w for w in msg.value.decode(errors="replace").split("\n") if w]
