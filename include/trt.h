@@ -93,7 +93,38 @@ typedef struct trt_render_opts {
   uint32_t seed;
   float ray_epsilon;
   float clamp;         /* firefly clamp on a contribution's max component; 0 = off */
+  uint32_t mode;       /* render mode (replaces Renderer::SetRenderMode, Tracer/Renderer/Renderer.h:48-49,
+                          with the kernels of Tracer/CUDA/GPU/Shade/*.h): TRT_MODE_* below */
+  uint32_t mode_flags; /* material property for TRT_MODE_MATERIAL_PROPERTY (the reference's
+                          MaterialPropertyIds order: anisotropic 0, clearcoat, clearcoat gloss, diffuse,
+                          emissive, metallic, normal, roughness, sheen, sheen tint, specular,
+                          specular tint, subsurface 12) */
+  float ao_dist;       /* ambient-occlusion distance (KernelSettings::aoDist) */
+  float zdepth_max;    /* depth shown as white by TRT_MODE_ZDEPTH (KernelSettings::zDepthMax) */
 } trt_render_opts;
+
+/* Render modes. Path tracing is 0; the others write one colour per camera sample
+ * (misses black), averaged like radiance. AO: fraction min(t, ao_dist) / ao_dist of a
+ * cosine-distributed ray from the first hit (AO_SHADING: times the base colour). Normals,
+ * tangent frames (built from the shading normal) and texture coordinates (barycentric
+ * (b1, b2, 0): the scene data has no UVs) are shown as (v + 1) / 2 or raw; ids as
+ * distinct colours; WIREFRAME marks hits near triangle edges; ZDEPTH = view depth /
+ * zdepth_max clamped to [0, 1]. */
+enum {
+  TRT_MODE_PATH_TRACING = 0,
+  TRT_MODE_AMBIENT_OCCLUSION = 1,
+  TRT_MODE_AMBIENT_OCCLUSION_SHADING = 2,
+  TRT_MODE_BITANGENT = 3,
+  TRT_MODE_GEOMETRIC_NORMAL = 4,
+  TRT_MODE_MATERIAL_ID = 5,
+  TRT_MODE_MATERIAL_PROPERTY = 6,
+  TRT_MODE_OBJECT_ID = 7,
+  TRT_MODE_SHADING_NORMAL = 8,
+  TRT_MODE_TANGENT = 9,
+  TRT_MODE_TEXTURE_COORDINATE = 10,
+  TRT_MODE_WIREFRAME = 11,
+  TRT_MODE_ZDEPTH = 12
+};
 
 /* Frame statistics (replaces RenderStatistics::DeviceStatistics,
  * Tracer/Renderer/RenderStatistics.h:11-26). Times are HIP-event milliseconds. */

@@ -29,6 +29,7 @@
 
 #include "trt/camera.h"
 #include "trt/integrator.h"
+#include "trt/render_modes.h"
 #include "trt/lights_build.h"
 
 using namespace trt;
@@ -476,6 +477,10 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
   rp.sample_offset = uopts[6];
   rp.eps = fopts[0];
   rp.clamp = fopts[1];
+  rp.mode = uopts[7];
+  rp.mode_flags = uopts[8];
+  rp.ao_dist = fopts[2];
+  rp.zdepth_max = fopts[3];
   const uint64_t npix = uint64_t(rp.width) * rp.height;
   const uint64_t n = npix * rp.spp;
   std::vector<float> L(n * 3, 0.f);
@@ -494,7 +499,7 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
         trace(*s, ps.o, ps.d, rp.eps, 1e30f, b, false, false);
         Hit h{b.t, b.b1, b.b2, b.prim, b.inst};
         ShadeOut out;
-        shade(s->view, s->samp.data(), rp, ps, h, d, out);
+        shade_mode(s->view, s->samp.data(), rp, ps, h, d, out);
         if (!is_black(out.L)) acc = acc + out.L;
         if (d == 0 && path < npix && aov) {
           float* a = aov + size_t(path) * 8;

@@ -164,8 +164,9 @@ class Oracle:
             r.update(render)
         cam = cam_array(camera or self.scene["camera"])
         W, H, spp, D = r["width"], r["height"], r["spp"], r["max_depth"]
-        uo = np.array([W, H, spp, D, r["rr_depth"], r["seed"], sample_offset], np.uint32)
-        fo = np.array([r["ray_epsilon"], r["clamp"]], np.float32)
+        uo = np.array([W, H, spp, D, r["rr_depth"], r["seed"], sample_offset, r.get("mode", 0),
+                       r.get("mode_flags", 0)], np.uint32)
+        fo = np.array([r["ray_epsilon"], r["clamp"], r.get("ao_dist", 1.0), r.get("zdepth_max", 1.0)], np.float32)
         paths = np.empty((W * H * spp, 3), np.float32)
         img32 = np.empty((H, W, 3), np.float32)
         img64 = np.empty((H, W, 3), np.float64)

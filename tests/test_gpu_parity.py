@@ -170,3 +170,17 @@ def test_path_radiance_with_sky(rend):
     _compare_paths(rend.path_radiance(), ref["path_rgb"], "sky")
     np.testing.assert_array_equal(rend.albedo()[..., :3], ref["aov"][..., 0:3])
     assert (ref["aov"][..., 0:3] > 0).all()  # every first hit or miss has an albedo (sky for misses)
+
+
+@pytest.mark.parametrize("mode", [m for m in range(1, 13)])
+def test_render_modes_match_oracle(rend, mode):
+    """Debug / AOV render modes: per-path values equal to the oracle's (bit-exact)."""
+    sc = scenes.cornell(width=40, height=32, spp=2, max_depth=4, seed=5)
+    sc["render"] = dict(sc["render"], mode=mode, mode_flags=7 if mode == 6 else 0, ao_dist=0.4, zdepth_max=3.0)
+    rend.UpdateScene(sc)
+    s = rend.settings
+    assert (s.mode, s.ao_dist) == (mode, 0.4)
+    rend.RenderFrame()
+    ref = _oracle(sc).render()
+    assert np.array_equal(rend.path_radiance().view(np.uint32), ref["path_rgb"].view(np.uint32)), mode
+    np.testing.assert_array_equal(rend.color()[..., :3], ref["image_f32"])

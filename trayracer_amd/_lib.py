@@ -77,6 +77,8 @@ class TrtRenderOpts(ctypes.Structure):
         ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
         ("max_depth", ctypes.c_uint32), ("rr_depth", ctypes.c_uint32), ("seed", ctypes.c_uint32),
         ("ray_epsilon", ctypes.c_float), ("clamp", ctypes.c_float),
+        ("mode", ctypes.c_uint32), ("mode_flags", ctypes.c_uint32), ("ao_dist", ctypes.c_float),
+        ("zdepth_max", ctypes.c_float),
     ]
 
 
@@ -181,6 +183,13 @@ PROTOTYPES = [
     ("trt_denoiser_activation", S, [P, ctypes.c_uint32, c_float_p, ctypes.c_uint64]),
     ("trt_denoiser_backward", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
 ]
+
+# render modes (TRT_MODE_* of include/trt.h)
+MODES = {"path_tracing": 0, "ambient_occlusion": 1, "ambient_occlusion_shading": 2, "bitangent": 3,
+         "geometric_normal": 4, "material_id": 5, "material_property": 6, "object_id": 7, "shading_normal": 8,
+         "tangent": 9, "texture_coordinate": 10, "wireframe": 11, "zdepth": 12}
+MATERIAL_PROPERTIES = ["anisotropic", "clearcoat", "clearcoatgloss", "diffuse", "emissive", "metallic", "normal",
+                       "roughness", "sheen", "sheentint", "specular", "speculartint", "subsurface"]
 
 STATUS = {0: "TRT_OK", 1: "TRT_INVALID_ARG", 2: "TRT_OUT_OF_MEMORY", 3: "TRT_DEVICE", 4: "TRT_NOT_BUILT",
           5: "TRT_UNSUPPORTED"}

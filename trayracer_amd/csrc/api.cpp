@@ -321,6 +321,10 @@ RenderParams params_of_opts(const trt_context* ctx, const trt_render_opts& o) {
   rp.sample_offset = ctx->sample_count;
   rp.eps = o.ray_epsilon;
   rp.clamp = o.clamp;
+  rp.mode = o.mode;
+  rp.mode_flags = o.mode_flags;
+  rp.ao_dist = o.ao_dist;
+  rp.zdepth_max = o.zdepth_max;
   return rp;
 }
 
@@ -350,7 +354,10 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
   ShadowQueue sq{ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>()};
   hipStream_t stm = ctx->stream;
   hipEvent_t* ev = ctx->ev;
-  const uint32_t D = std::min<uint32_t>(o.max_depth, kMaxDepth);
+  // debug modes: one bounce (two for ambient occlusion)
+  const uint32_t D = o.mode == kModePathTracing ? std::min<uint32_t>(o.max_depth, kMaxDepth)
+                     : std::min<uint32_t>(o.max_depth, (o.mode == kModeAmbientOcclusion ||
+                                                        o.mode == kModeAmbientOcclusionShading) ? 2u : 1u);
 
   const uint32_t cap = ctx->seg_cap;
   launch_zero_counters(ctr, stm);
@@ -748,6 +755,13 @@ static trt_status check_render(trt_context* ctx, const trt_render_opts* o) {
   if (!(o->ray_epsilon >= 0.f) || !(o->clamp >= 0.f)) return fail(ctx, TRT_INVALID_ARG, "invalid epsilon/clamp");
   if (uint64_t(o->width) * o->height * o->spp >= (1ull << 30))
     return fail(ctx, TRT_INVALID_ARG, "width*height*spp must be below 2^30");
+  if (o->mode >= kNumRenderModes) return fail(ctx, TRT_INVALID_ARG, "unknown render mode");
+  if ((o->mode == kModeAmbientOcclusion || o->mode == kModeAmbientOcclusionShading) && !(o->ao_dist > 0.f))
+    return fail(ctx, TRT_INVALID_ARG, "ambient occlusion needs ao_dist > 0");
+  if (o->mode == kModeZDepth && !(o->zdepth_max > 0.f))
+    return fail(ctx, TRT_INVALID_ARG, "z-depth mode needs zdepth_max > 0");
+  if (o->mode == kModeMaterialProperty && o->mode_flags >= kNumMaterialProperties)
+    return fail(ctx, TRT_INVALID_ARG, "unknown material property");
   return TRT_OK;
 }
 
@@ -913,6 +927,7 @@ trt_status trt_backward(trt_context* ctx, const trt_render_opts* o, const float*
       L.ray_epsilon != o->ray_epsilon || L.clamp != o->clamp)
     return fail(ctx, TRT_INVALID_ARG, "trt_backward replays the last trt_render: call it with the same options");
   if (o->clamp != 0.f) return fail(ctx, TRT_UNSUPPORTED, "the adjoint needs clamp = 0");
+  if (o->mode != kModePathTracing) return fail(ctx, TRT_UNSUPPORTED, "the adjoint needs the path-tracing mode");
   HIPC(hipSetDevice(ctx->device), "hipSetDevice");
   const uint32_t W = o->width, H = o->height, D = std::min<uint32_t>(o->max_depth, kMaxDepth);
   const uint64_t n = uint64_t(W) * H * o->spp;

@@ -6,6 +6,7 @@
 
 #include "bvh_build.h"
 #include "trt/integrator.h"
+#include "trt/render_modes.h"
 
 namespace trt {
 

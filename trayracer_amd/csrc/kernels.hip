@@ -760,6 +760,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevAccel acc, ShadowQueue sq,
   }
 }
 
+// DEBUG: the render modes of render_modes.h instead of the path-tracing step.
+template <bool DEBUG>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* sampling_mats, RenderParams rp,
                                                    RayQueue q, HitBuf h, RayQueue qn, ShadowQueue sq,
                                                    FrameBuffers fb, SegIn in, uint32_t* next_cnt,
@@ -796,7 +798,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* 
       hit.b2 = hh.z;
       hit.prim = __float_as_uint(hh.w);
       hit.inst = two_level ? h.inst[slot] : kNoInstance;
-      shade(sv, sampling_mats, rp, ps, hit, depth, out);
+      if (DEBUG) shade_debug(sv, rp, ps, hit, depth, out);
+      else shade(sv, sampling_mats, rp, ps, hit, depth, out);
       if (!is_black(out.L)) {
         float4 L = fb.radiance[path];
         L.x += out.L.x;
@@ -1169,8 +1172,12 @@ void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t
 void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
                   HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, SegIn in, uint32_t* next_cnt,
                   uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, fb, in,
-                     next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u);
+  if (rp.mode != kModePathTracing)
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, fb, in,
+                       next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, fb,
+                       in, next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u);
 }
 
 void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,

@@ -32,9 +32,13 @@ struct alignas(16) RenderParams {
   CameraParams cam;
   uint32_t width, height, spp, max_depth;
   uint32_t rr_depth, seed, sample_offset, flags;
-  float eps;       // ray epsilon (reference KernelSettings::rayEpsilon)
-  float clamp;     // firefly clamp on the max component of a contribution; 0 = off
-  float pad0, pad1;
+  float eps;        // ray epsilon (reference KernelSettings::rayEpsilon)
+  float clamp;      // firefly clamp on the max component of a contribution; 0 = off
+  float ao_dist;    // ambient-occlusion distance (KernelSettings::aoDist)
+  float zdepth_max; // depth of white in the z-depth mode (KernelSettings::zDepthMax)
+  uint32_t mode;    // render mode (render_modes.h); 0 = path tracing
+  uint32_t mode_flags;  // material property shown by the material-property mode
+  uint32_t pad0, pad1;
 };
 
 struct PathState {

@@ -112,6 +112,7 @@ struct SurfacePoint {
   V3 ng;      // world geometric normal, facing the incoming ray
   V3 ns;      // world shading normal, same side as ng
   V3 ng_raw;  // world geometric normal by winding (emission side)
+  V3 ns_raw;  // world shading normal before it is turned to the geometric normal's side
   float area; // world-space area
   uint32_t mat;
 };
@@ -157,7 +158,8 @@ TRT_HD inline SurfacePoint surface_at(const SceneView& s, const Hit& h, V3 ray_o
   (void)ray_o;
   V3 ng = sp.ng_raw;
   if (dot(ng, d) > 0.f) ng = -ng;
-  V3 ns = s.tri_nrm ? normalize(n_sh) : ng;
+  V3 ns = s.tri_nrm ? normalize(n_sh) : sp.ng_raw;
+  sp.ns_raw = ns;
   if (dot(ns, ng) < 0.f) ns = -ns;
   sp.ng = ng;
   sp.ns = ns;

@@ -26,16 +26,19 @@ class KernelSettings:
     """Reference KernelSettings (Tracer/Common/CommonStructs.h:145-153) plus the film
     size, Russian-roulette start depth and firefly clamp."""
 
-    def __init__(self, width=256, height=256, spp=1, max_depth=4, rr_depth=3, seed=0, ray_epsilon=1e-3, clamp=0.0):
+    def __init__(self, width=256, height=256, spp=1, max_depth=4, rr_depth=3, seed=0, ray_epsilon=1e-3, clamp=0.0,
+                 mode=0, mode_flags=0, ao_dist=1.0, zdepth_max=1.0):
         self.width, self.height, self.spp = width, height, spp
         self.max_depth, self.rr_depth, self.seed = max_depth, rr_depth, seed
         self.ray_epsilon, self.clamp = ray_epsilon, clamp
+        self.mode, self.mode_flags, self.ao_dist, self.zdepth_max = mode, mode_flags, ao_dist, zdepth_max
 
     def to_c(self):
         o = _lib.TrtRenderOpts()
         o.width, o.height, o.spp = self.width, self.height, self.spp
         o.max_depth, o.rr_depth, o.seed = self.max_depth, self.rr_depth, self.seed
         o.ray_epsilon, o.clamp = self.ray_epsilon, self.clamp
+        o.mode, o.mode_flags, o.ao_dist, o.zdepth_max = self.mode, self.mode_flags, self.ao_dist, self.zdepth_max
         return o
 
     @classmethod
@@ -130,7 +133,16 @@ class Renderer:
         self._check(self.lib.trt_load_scene_file(self.ctx, os.fsencode(path), ctypes.byref(bo), ctypes.byref(info)))
         s = self.settings
         s.spp, s.max_depth, s.ray_epsilon = info.spp, info.max_depth, info.ray_epsilon
+        s.ao_dist, s.zdepth_max = info.ao_dist, info.zdepth_max
         return info.as_dict()
+
+    def SetRenderMode(self, mode, flags=0):
+        """Render mode by name or number (Renderer::SetRenderMode; _lib.MODES) and, for
+        the material-property mode, the property (name from _lib.MATERIAL_PROPERTIES or id)."""
+        m = _lib.MODES[mode] if isinstance(mode, str) else int(mode)
+        f = _lib.MATERIAL_PROPERTIES.index(flags) if isinstance(flags, str) else int(flags)
+        self.settings.mode, self.settings.mode_flags = m, f
+        self.Reset()
 
     def bvh_info(self):
         info = _lib.TrtBvhInfo()

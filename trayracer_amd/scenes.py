@@ -307,6 +307,7 @@ def from_file(path, width=256, height=256, rr_depth=3, seed=0):
     scene = dict(positions=pos, normals=nrm, mat_ids=mids, meshes=[(int(o), int(c)) for o, c in zip(toff, tcnt)],
                  materials=mat_arr, instances=inst_arr, camera=cam, sky=sky,
                  render=dict(width=width, height=height, spp=int(info.spp), max_depth=int(info.max_depth),
-                             rr_depth=rr_depth, seed=seed, ray_epsilon=float(info.ray_epsilon), clamp=0.0))
+                             rr_depth=rr_depth, seed=seed, ray_epsilon=float(info.ray_epsilon), clamp=0.0,
+                             ao_dist=float(info.ao_dist), zdepth_max=float(info.zdepth_max)))
     warnings = [w for w in msg.value.decode(errors="replace").split("\n") if w]
     return scene, d, warnings
