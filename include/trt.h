@@ -237,6 +237,25 @@ trt_status trt_scene_file_read(const char* path, trt_scene_file_info* info, floa
 trt_status trt_load_scene_file(trt_context* ctx, const char* path, const trt_build_opts* opts,
                                trt_scene_file_info* info);
 
+/* Display transform (replaces the tone-mapping post pass, Tracer/glsl/Tonemap.frag, with
+ * its uniforms exposure / gamma / tonemapMethod, Tracer/glsl/Tonemap.json): colour x
+ * exposure, tone curve `method` (0 ACES, 1 filmic, 2 Lottes, 3 Reinhard, 4 Reinhard2,
+ * 5 Uchimura, 6 Uncharted 2, other: none), gamma, 8-bit RGBA (alpha 255), rows bottom-up
+ * like the colour buffer. source 0: the mean colour, 1: the last denoised frame. */
+typedef struct trt_tonemap_opts {
+  uint32_t method;
+  float exposure;  /* 1 */
+  float gamma;     /* 2.2 */
+  uint32_t source;
+} trt_tonemap_opts;
+trt_status trt_tonemap(trt_context* ctx, const trt_tonemap_opts* opts, uint8_t* rgba8);
+
+/* Image files (replaces the screenshot/export path of the reference's GUI, which saves
+ * through FreeImage): portable float map (RGB, bottom-up rows as stored) and PNG (8-bit
+ * RGBA, written top-down). Buffers are width x height x 4, rows bottom-up. No context. */
+trt_status trt_write_pfm(const char* path, uint32_t width, uint32_t height, const float* rgba);
+trt_status trt_write_png(const char* path, uint32_t width, uint32_t height, const uint8_t* rgba8);
+
 /* Sky seen by rays that leave the scene (replaces Sky and SampleSky,
  * Tracer/Renderer/Sky.h:10-66, Tracer/CUDA/GPU/CudaSky.h:44-156): the reference's
  * Preetham-style daylight model. Not sampled by next-event estimation (the reference

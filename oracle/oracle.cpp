@@ -30,6 +30,7 @@
 #include "trt/camera.h"
 #include "trt/integrator.h"
 #include "trt/render_modes.h"
+#include "trt/tonemap.h"
 #include "trt/lights_build.h"
 
 using namespace trt;
@@ -412,6 +413,21 @@ void orc_sky_radiance(void* p, const float* dirs, uint32_t n, uint32_t sun, floa
     out[i * 3 + 1] = r.y;
     out[i * 3 + 2] = r.z;
   }
+}
+
+// Display transform of n RGB(A) pixels (stride floats apart) -> RGBA8.
+void orc_tonemap(const float* rgb, uint32_t n, uint32_t stride, uint32_t method, float exposure, float gamma,
+                 uint8_t* out) {
+  const TonemapParams tp{method, exposure, gamma};
+  for (uint32_t i = 0; i < n; ++i) {
+    for (int c = 0; c < 3; ++c) out[i * 4 + c] = uint8_t(tonemap_u8(tp, rgb[size_t(i) * stride + c]));
+    out[i * 4 + 3] = 255;
+  }
+}
+
+// Tone curve alone (before gamma and quantisation), for tests.
+void orc_tone_curve(const float* x, uint32_t n, uint32_t method, float* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = tonemap_channel(method, x[i]);
 }
 
 void orc_scene_destroy(void* p) { delete static_cast<OScene*>(p); }

@@ -25,6 +25,9 @@ _PROTO = [
     ("orc_trace_any", None, [_P, _f, ctypes.c_uint32, _u8, ctypes.c_int, ctypes.c_int]),
     ("orc_render", None, [_P, _f, _u, _f, _f, _f, _d, _f, _u64, ctypes.c_int]),
     ("orc_set_material_params", None, [_P, _f, ctypes.c_uint32, ctypes.c_uint32]),
+    ("orc_tonemap", None, [_f, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                           ctypes.POINTER(ctypes.c_uint8)]),
+    ("orc_tone_curve", None, [_f, ctypes.c_uint32, ctypes.c_uint32, _f]),
     ("orc_set_sky", None, [_P, ctypes.c_uint32, ctypes.c_uint32, _f, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     ("orc_sky_radiance", None, [_P, _f, ctypes.c_uint32, ctypes.c_uint32, _f]),
     ("orc_backward", None, [_P, _f, _u, _f, _f, _d, ctypes.c_int]),
@@ -311,4 +314,21 @@ def camera_rays(cam, W, H, px, u):
     u = np.ascontiguousarray(u, np.float32).reshape(-1, 5)
     out = np.empty((px.shape[0], 7), np.float32)
     load_oracle().orc_camera_rays(fp(c), W, H, up(px), fp(u), px.shape[0], fp(out))
+    return out
+
+
+def tonemap(img, method=0, exposure=1.0, gamma=2.2):
+    """img [..., 3 or 4] float32 -> uint8 [..., 4] through the shared display transform."""
+    a = np.ascontiguousarray(img, np.float32)
+    stride = a.shape[-1]
+    n = a.size // stride
+    out = np.empty(a.shape[:-1] + (4,), np.uint8)
+    load_oracle().orc_tonemap(fp(a), n, stride, method, exposure, gamma, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out
+
+
+def tone_curve(x, method):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    load_oracle().orc_tone_curve(fp(x), x.size, method, fp(out))
     return out

@@ -184,3 +184,35 @@ def test_render_modes_match_oracle(rend, mode):
     ref = _oracle(sc).render()
     assert np.array_equal(rend.path_radiance().view(np.uint32), ref["path_rgb"].view(np.uint32)), mode
     np.testing.assert_array_equal(rend.color()[..., :3], ref["image_f32"])
+
+
+@pytest.mark.parametrize("lens", [dict(aperture=0.05, focal_dist=1.6, bokeh_sides=0, bokeh_rotation=0.0, distortion=0.0),
+                                  dict(aperture=0.08, focal_dist=1.2, bokeh_sides=6, bokeh_rotation=0.4,
+                                       distortion=0.25)],
+                         ids=["thin-lens disc", "hexagonal bokeh + distortion"])
+def test_camera_lens_models(rend, lens):
+    """Thin lens (disc or polygonal aperture) and barrel distortion: per-path radiance and
+    AOVs equal to the oracle's."""
+    sc = scenes.cornell(width=40, height=32, spp=4, max_depth=4, seed=21)
+    sc["camera"] = dict(sc["camera"], **lens)
+    rend.UpdateScene(sc)
+    rend.RenderFrame()
+    ref = _oracle(sc).render()
+    _compare_paths(rend.path_radiance(), ref["path_rgb"], "lens")
+    np.testing.assert_array_equal(rend.normal(), ref["aov"][..., 4:8])
+
+
+def test_tonemap_matches_host_transform(rend, tmp_path):
+    """Device display transform (all curves, colour and denoised sources) equals the
+    host transform of the same floats; PNG export of the result."""
+    from oracle.oracle_py import tonemap
+    sc = scenes.cornell(width=64, height=48, spp=4, max_depth=4, seed=2)
+    rend.UpdateScene(sc)
+    rend.RenderFrame()
+    col = rend.color()
+    for method in range(8):
+        for exposure, gamma in ((1.0, 2.2), (3.5, 1.8)):
+            got = rend.tonemap(method, exposure, gamma)
+            np.testing.assert_array_equal(got, tonemap(col, method, exposure, gamma))
+    rend.save_png(str(tmp_path / "c.png"), method="aces")
+    assert (tmp_path / "c.png").stat().st_size > 64 * 48 * 4

@@ -67,6 +67,11 @@ class TrtSky(ctypes.Structure):
                 ("angular_diameter", ctypes.c_float), ("intensity", ctypes.c_float), ("turbidity", ctypes.c_float)]
 
 
+class TrtTonemapOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_uint32), ("exposure", ctypes.c_float), ("gamma", ctypes.c_float),
+                ("source", ctypes.c_uint32)]
+
+
 class TrtBuildOpts(ctypes.Structure):
     _fields_ = [("bins", ctypes.c_int32), ("max_leaf", ctypes.c_int32), ("threads", ctypes.c_int32),
                 ("presplit", ctypes.c_float)]
@@ -179,6 +184,9 @@ PROTOTYPES = [
                                c_float_p]),
     ("trt_denoise", S, [P, ctypes.POINTER(TrtDenoiseStats)]),
     ("trt_read_denoised", S, [P, c_float_p]),
+    ("trt_tonemap", S, [P, ctypes.POINTER(TrtTonemapOpts), c_u8_p]),
+    ("trt_write_pfm", S, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
+    ("trt_write_png", S, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, c_u8_p]),
     ("trt_denoiser_forward", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
     ("trt_denoiser_activation", S, [P, ctypes.c_uint32, c_float_p, ctypes.c_uint64]),
     ("trt_denoiser_backward", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),

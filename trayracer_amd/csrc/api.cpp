@@ -106,6 +106,7 @@ struct trt_context {
   DevBuf s_o, s_d, s_c;
   DevBuf radiance;
   DevBuf accum, color, albedo, normal;
+  DevBuf tonemapped;  // RGBA8 display image
   DevBuf counters;
   DevBuf scratch_rays, scratch_hits, scratch_inst, scratch_occ;
   uint64_t path_capacity = 0;
@@ -1170,6 +1171,29 @@ trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* st) {
     (void)hipEventElapsedTime(&st->network_ms, ev[1], ev[2]);
     (void)hipEventElapsedTime(&st->total_ms, ev[0], ev[2]);
   }
+  return TRT_OK;
+}
+
+trt_status trt_tonemap(trt_context* ctx, const trt_tonemap_opts* opts, uint8_t* rgba8) {
+  if (!ctx || !opts || !rgba8) return TRT_INVALID_ARG;
+  if (!(opts->exposure >= 0.f) || !(opts->exposure < 1e30f) || !(opts->gamma > 0.f) || !(opts->gamma < 1e6f))
+    return fail(ctx, TRT_INVALID_ARG, "exposure must be >= 0 and gamma > 0");
+  const uint64_t npix = uint64_t(ctx->width) * ctx->height;
+  if (npix == 0) return fail(ctx, TRT_NOT_BUILT, "no frame");
+  const DevBuf* src = &ctx->color;
+  if (opts->source == 1) {
+    if (!ctx->dn_denoised.p || ctx->dn_denoised.bytes < npix * 16) return fail(ctx, TRT_NOT_BUILT, "no denoised frame");
+    src = &ctx->dn_denoised;
+  } else if (opts->source != 0) {
+    return fail(ctx, TRT_INVALID_ARG, "source must be 0 (colour) or 1 (denoised)");
+  }
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPC(ctx->tonemapped.ensure(npix * 4), "alloc tonemap");
+  TonemapParams tp{opts->method, opts->exposure, opts->gamma};
+  launch_tonemap(src->as<float4>(), uint32_t(npix), tp, ctx->tonemapped.as<uchar4>(), ctx->stream);
+  HIPC(hipGetLastError(), "tonemap launch");
+  HIPC(hipMemcpyAsync(rgba8, ctx->tonemapped.p, npix * 4, hipMemcpyDeviceToHost, ctx->stream), "read tonemap");
+  HIPC(hipStreamSynchronize(ctx->stream), "tonemap");
   return TRT_OK;
 }
 

@@ -7,6 +7,7 @@
 #include "bvh_build.h"
 #include "trt/integrator.h"
 #include "trt/render_modes.h"
+#include "trt/tonemap.h"
 
 namespace trt {
 
@@ -134,6 +135,7 @@ void launch_shade(const SceneView& sv, const Material* sampling_mats, const Rend
 void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,
                    bool stats, hipStream_t stream);
 void launch_sum_counts(uint32_t* ctr, uint32_t cap, hipStream_t stream);
+void launch_tonemap(const float4* src, uint32_t n, const TonemapParams& tp, uchar4* dst, hipStream_t stream);
 void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
                      hipStream_t stream);
 void launch_raygen_adj(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, const float4* radiance,

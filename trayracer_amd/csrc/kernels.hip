@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device.h"
+#include "trt/tonemap.h"
 
 namespace trt {
 
@@ -856,6 +857,15 @@ __global__ __launch_bounds__(kBlock) void k_finalize(RenderParams rp, const floa
   fb.color[pix] = make_float4(acc.x * inv, acc.y * inv, acc.z * inv, 1.f);
 }
 
+__global__ __launch_bounds__(kBlock) void k_tonemap(const float4* src, uint32_t n, TonemapParams tp,
+                                                     uchar4* dst) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 c = src[i];
+  dst[i] = make_uchar4(static_cast<unsigned char>(tonemap_u8(tp, c.x)), static_cast<unsigned char>(tonemap_u8(tp, c.y)),
+                       static_cast<unsigned char>(tonemap_u8(tp, c.z)), 255);
+}
+
 __global__ __launch_bounds__(kBlock) void k_trace(DevAccel acc, const float4* rays, uint32_t n, float4* hits,
                                                    uint32_t* hit_inst, uint32_t* occluded, uint32_t any,
                                                    uint32_t* ctr) {
@@ -1140,6 +1150,10 @@ int trace_blocks_per_cu(const DevAccel& acc) {
 
 void launch_zero_counters(uint32_t* ctr, hipStream_t stream) {
   hipLaunchKernelGGL(k_zero_counters, dim3((kNumCounters + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, ctr);
+}
+
+void launch_tonemap(const float4* src, uint32_t n, const TonemapParams& tp, uchar4* dst, hipStream_t stream) {
+  hipLaunchKernelGGL(k_tonemap, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, src, n, tp, dst);
 }
 
 void launch_sum_counts(uint32_t* ctr, uint32_t cap, hipStream_t stream) {

@@ -136,6 +136,28 @@ class Renderer:
         s.ao_dist, s.zdepth_max = info.ao_dist, info.zdepth_max
         return info.as_dict()
 
+    TONEMAP = {"aces": 0, "filmic": 1, "lottes": 2, "reinhard": 3, "reinhard2": 4, "uchimura": 5, "uncharted2": 6,
+               "none": 7}
+
+    def tonemap(self, method="aces", exposure=1.0, gamma=2.2, denoised=False):
+        """8-bit RGBA display image [H, W, 4] (row 0 = bottom) of the mean colour or the
+        denoised frame (Tonemap.frag: exposure, curve, gamma)."""
+        m = self.TONEMAP[method] if isinstance(method, str) else int(method)
+        o = _lib.TrtTonemapOpts(m, exposure, gamma, 1 if denoised else 0)
+        s = self.settings
+        out = np.empty((s.height, s.width, 4), np.uint8)
+        self._check(self.lib.trt_tonemap(self.ctx, ctypes.byref(o), out.ctypes.data_as(_lib.c_u8_p)))
+        return out
+
+    def save_png(self, path, **tonemap_args):
+        img = self.tonemap(**tonemap_args)
+        self._check(self.lib.trt_write_png(os.fsencode(path), img.shape[1], img.shape[0],
+                                           img.ctypes.data_as(_lib.c_u8_p)))
+
+    def save_pfm(self, path, denoised=False):
+        img = np.ascontiguousarray(self.denoised() if denoised else self.color(), np.float32)
+        self._check(self.lib.trt_write_pfm(os.fsencode(path), img.shape[1], img.shape[0], _fp(img)))
+
     def SetRenderMode(self, mode, flags=0):
         """Render mode by name or number (Renderer::SetRenderMode; _lib.MODES) and, for
         the material-property mode, the property (name from _lib.MATERIAL_PROPERTIES or id)."""
