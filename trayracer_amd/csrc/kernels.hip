@@ -52,51 +52,22 @@ __device__ __forceinline__ void wave_add_stat(unsigned long long* dst, uint32_t 
   if (lane_id() == 0 && s) atomicAdd(dst, static_cast<unsigned long long>(s));
 }
 
-// Fetches the next batch of up to 64 items of a segmented queue for the wave, starting
-// with segment `home`. Returns false when every segment is exhausted. `n_seg` holds,
-// in lane s < kSegs, the item count of segment s (constant during the kernel); `done`
-// collects segments found exhausted (work cursors only grow, so they stay exhausted):
-// every wave makes at most kSegs failed attempts, so the loop always ends.
-__device__ __forceinline__ bool seg_fetch(const SegIn& in, uint32_t n_seg, uint32_t home, uint64_t& done,
-                                          uint32_t& slot0, uint32_t& n_valid) {
-  const uint32_t lane = lane_id();
-  for (;;) {
-    uint32_t rem = 0;
-    if (lane < kSegs && !((done >> lane) & 1ull)) {
-      const uint32_t w = __hip_atomic_load(&in.work[lane * kLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      rem = w < n_seg ? n_seg - w : 0u;
-    }
-    const uint64_t m = __ballot(rem > 0u);
-    if (m == 0ull) return false;
-    const uint64_t rot = ((m >> home) | (m << (kSegs - home))) & ((1ull << kSegs) - 1ull);
-    uint32_t sgm = home + static_cast<uint32_t>(__builtin_ctzll(rot));
-    if (sgm >= kSegs) sgm -= kSegs;
-    uint32_t b = 0;
-    if (lane == 0) b = atomicAdd(&in.work[sgm * kLine], 64u);
-    b = __shfl(b, 0);
-    const uint32_t ns = __shfl(n_seg, static_cast<int>(sgm));
-    if (b < ns) {
-      slot0 = sgm * in.cap + b;
-      n_valid = ns - b < 64u ? ns - b : 64u;
-      return true;
-    }
-    done |= 1ull << sgm;
-  }
-}
-
-// Like seg_fetch, for up to `want` items (1..64) from one segment.
+// Fetches the next batch of up to `want` (1..64) items of a segmented queue for the
+// wave: one atomic on the fetch cursor of the first segment, from `home` on, not yet
+// found exhausted. `n_seg` holds, in lane s < kSegs, the item count of segment s
+// (constant during the kernel); `done` collects exhausted segments (cursors only grow,
+// so they stay exhausted; callers start it with seg_empty()). Every wave makes at most
+// kSegs failed attempts, so the loop always ends. The cursors are not pre-read: coherent
+// reads of a hot line serialise like the atomics themselves (measured: a pre-read of all
+// 32 cursors per batch cost as much as the whole Shade kernel's work).
 __device__ __forceinline__ bool seg_fetch_k(const SegIn& in, uint32_t n_seg, uint32_t home, uint64_t& done,
                                             uint32_t want, uint32_t& slot0, uint32_t& n_got) {
   const uint32_t lane = lane_id();
+  const uint64_t all = (1ull << kSegs) - 1ull;
   for (;;) {
-    uint32_t rem = 0;
-    if (lane < kSegs && !((done >> lane) & 1ull)) {
-      const uint32_t w = __hip_atomic_load(&in.work[lane * kLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      rem = w < n_seg ? n_seg - w : 0u;
-    }
-    const uint64_t m = __ballot(rem > 0u);
+    const uint64_t m = ~done & all;
     if (m == 0ull) return false;
-    const uint64_t rot = ((m >> home) | (m << (kSegs - home))) & ((1ull << kSegs) - 1ull);
+    const uint64_t rot = ((m >> home) | (m << (kSegs - home))) & all;
     uint32_t sgm = home + static_cast<uint32_t>(__builtin_ctzll(rot));
     if (sgm >= kSegs) sgm -= kSegs;
     uint32_t b = 0;
@@ -110,6 +81,16 @@ __device__ __forceinline__ bool seg_fetch_k(const SegIn& in, uint32_t n_seg, uin
     }
     done |= 1ull << sgm;
   }
+}
+
+__device__ __forceinline__ bool seg_fetch(const SegIn& in, uint32_t n_seg, uint32_t home, uint64_t& done,
+                                          uint32_t& slot0, uint32_t& n_valid) {
+  return seg_fetch_k(in, n_seg, home, done, 64u, slot0, n_valid);
+}
+
+// Segments without items: the initial `done` mask of seg_fetch (uniform per wave).
+__device__ __forceinline__ uint64_t seg_empty(uint32_t n_seg) {
+  return ~__ballot(n_seg > 0u) & ((1ull << kSegs) - 1ull);
 }
 
 // Per-lane item count of segment `lane` (lanes >= kSegs: 0).
@@ -390,7 +371,7 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
   const int cap = acc.stack_cap;
   const DevNode2* __restrict__ nodes = acc.nodes;
   const float4* __restrict__ tris = acc.tris;
-  uint64_t seg_done = 0;
+  uint64_t seg_done = seg_empty(n_seg);
   bool drained = false;
   uint32_t slot = kNoHit;
   V3 o = v3(0.f), d = v3(0.f);
@@ -634,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevAccel acc, RayQueue q, Hit
   const uint32_t lane = lane_id();
   uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
   const uint32_t n_seg = seg_counts(in), home = wave_home();
-  uint64_t done = 0;
+  uint64_t done = seg_empty(n_seg);
   uint32_t slot0, n_valid;
   while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
     const uint32_t slot = slot0 + lane;
@@ -729,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevAccel acc, ShadowQueue sq,
   const uint32_t lane = lane_id();
   uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
   const uint32_t n_seg = seg_counts(in), home = wave_home();
-  uint64_t done = 0;
+  uint64_t done = seg_empty(n_seg);
   uint32_t slot0, n_valid;
   while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
     const uint32_t slot = slot0 + lane;
@@ -771,7 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* 
   const uint32_t lane = lane_id();
   const uint32_t npix = rp.width * rp.height;
   const uint32_t n_seg = seg_counts(in), home = wave_home();
-  uint64_t done = 0;
+  uint64_t done = seg_empty(n_seg);
   uint32_t slot0, n_valid;
   while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
     const uint32_t slot = slot0 + lane;
@@ -779,12 +760,20 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* 
     out.has_ext = 0;
     out.has_shadow = 0;
     uint32_t path = 0;
-    const float4 b = lane < n_valid ? q.d_tmax[slot] : make_float4(0.f, 0.f, 0.f, -1.f);
+    // all loads of the slot at once (a dead primary ray has tmax < 0 and is skipped)
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 0.f, -1.f), tp = a, hh = a;
+    uint32_t meta = 0, hinst = kNoInstance;
+    if (lane < n_valid) {
+      a = q.o_tmin[slot];
+      b = q.d_tmax[slot];
+      tp = q.t_pdf[slot];
+      meta = q.meta[slot];
+      hh = h.thit[slot];
+      if (two_level) hinst = h.inst[slot];
+    }
     if (b.w >= 0.f) {
-      const float4 a = q.o_tmin[slot];
-      const float4 tp = q.t_pdf[slot];
-      const uint32_t meta = q.meta[slot];
-      const float4 hh = h.thit[slot];
+      // this lane is the only writer of radiance[path] in this kernel: read it early
+      const float4 L0 = fb.radiance[meta >> 2];
       PathState ps;
       ps.o = v3(a.x, a.y, a.z);
       ps.d = v3(b.x, b.y, b.z);
@@ -798,11 +787,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* 
       hit.b1 = hh.y;
       hit.b2 = hh.z;
       hit.prim = __float_as_uint(hh.w);
-      hit.inst = two_level ? h.inst[slot] : kNoInstance;
+      hit.inst = hinst;
       if (DEBUG) shade_debug(sv, rp, ps, hit, depth, out);
       else shade(sv, sampling_mats, rp, ps, hit, depth, out);
       if (!is_black(out.L)) {
-        float4 L = fb.radiance[path];
+        float4 L = L0;
         L.x += out.L.x;
         L.y += out.L.y;
         L.z += out.L.z;
@@ -929,7 +918,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneView sv, const Materi
   const float inv_spp = 1.f / static_cast<float>(rp.spp);
   const size_t R = ab.n_records;
   const uint32_t n_seg = seg_counts(in), home = wave_home();
-  uint64_t done = 0;
+  uint64_t done = seg_empty(n_seg);
   uint32_t slot0, n_valid;
   while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
     const uint32_t slot = slot0 + lane;
@@ -1032,7 +1021,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_adj(DevAccel acc, ShadowQueue
   uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
   const size_t R = ab.n_records;
   const uint32_t n_seg = seg_counts(in), home = wave_home();
-  uint64_t done = 0;
+  uint64_t done = seg_empty(n_seg);
   uint32_t slot0, n_valid;
   while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
     const uint32_t slot = slot0 + lane;

@@ -131,6 +131,18 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
     return;
   }
 
+  // The light pick depends only on the vertex's random numbers: draw them and start the
+  // alias-table and light-record loads before the surface's, so that on the GPU the two
+  // chains of dependent loads overlap (values are unchanged by the order).
+  uint32_t pixel, sample;
+  path_pixel_sample(rp, ps.path, pixel, sample);
+  Pcg32 rng = rng_for(rp.seed, pixel, sample, depth);
+  float u[8];
+  for (int i = 0; i < 8; ++i) u[i] = rng_float(rng);
+  const bool nee = s.n_lights > 0 && s.total_power > 0.f;
+  LightTri L{};
+  if (nee) L = s.lights[alias_sample(s.alias_prob, s.alias_idx, s.n_lights, u[0], u[1])];
+
   const SurfacePoint sp = surface_at(s, h, ps.o, ps.d);
   const Material& m = s.materials[sp.mat];
   const Material& ms = sampling_mats[sp.mat];
@@ -154,12 +166,6 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
     }
     return;  // paths end at emitters
   }
-
-  uint32_t pixel, sample;
-  path_pixel_sample(rp, ps.path, pixel, sample);
-  Pcg32 rng = rng_for(rp.seed, pixel, sample, depth);
-  float u[8];
-  for (int i = 0; i < 8; ++i) u[i] = rng_float(rng);
 
   const Frame fr = make_frame(sp.ns);
   const V3 wo = to_local(fr, -ps.d);
@@ -186,9 +192,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
   }
 
   // next-event estimation
-  if (s.n_lights > 0 && s.total_power > 0.f) {
-    const uint32_t li = alias_sample(s.alias_prob, s.alias_idx, s.n_lights, u[0], u[1]);
-    const LightTri& L = s.lights[li];
+  if (nee) {
     float b1, b2;
     sample_triangle(u[2], u[3], b1, b2);
     const V3 p0 = v3(L.p0[0], L.p0[1], L.p0[2]);
