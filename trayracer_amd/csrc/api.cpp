@@ -85,6 +85,7 @@ struct trt_context {
   // device scene
   DevBuf d_pos, d_nrm, d_mat_ids, d_materials, d_inst, d_inst_tlas, d_lights, d_alias_p, d_alias_i;
   DevBuf d_nodes, d_tris, d_tlas;
+  DevBuf d_spill;  // traversal stack spill areas
   uint32_t n_lights = 0;
   float total_power = 0.f;
   bool use_normals = false;
@@ -285,6 +286,7 @@ DevAccel accel_of(const trt_context* ctx) {
   a.inst = ctx->two_level ? ctx->d_inst_tlas.as<Instance>() : nullptr;
   a.stack_cap = ctx->stack_cap;
   a.two_level = ctx->two_level ? 1 : 0;
+  a.spill = ctx->d_spill.as<uint32_t>();
   return a;
 }
 
@@ -705,6 +707,8 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   // persistent grid sizes from occupancy (every resident slot gets a wave)
   ctx->grid_trace = ctx->num_cus * trace_blocks_per_cu(accel_of(ctx));
   ctx->grid_shade = ctx->num_cus * 8;
+  // stack spill areas of the single-level traversal: stack_cap entries per grid thread
+  HIPC(ctx->d_spill.ensure(size_t(ctx->grid_trace) * 256 * size_t(ctx->stack_cap) * 4), "alloc stack spill");
 
   const auto t1 = std::chrono::steady_clock::now();
   ctx->info = trt_bvh_info{};

@@ -39,8 +39,10 @@ struct DevAccel {
   const float4* tris;       // reordered traversal triangles: 3 float4 per triangle (w of row 0 = prim id)
   const DevNode2* tlas;     // top level over instances (nullptr for single-level)
   const Instance* inst;     // instances (two-level)
-  int stack_cap;            // LDS stack entries per lane
+  int stack_cap;            // stack entries per lane (tree depth bound)
   int two_level;
+  uint32_t* spill;          // single-level kernels: per-lane stack spill area, stack_cap entries
+                            // per thread of the traversal grid (blockIdx.x * 256 + threadIdx.x)
 };
 
 // Queues are split into kSegs segments of `cap` slots each (slot = segment * cap + i).

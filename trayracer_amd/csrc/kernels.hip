@@ -357,11 +357,44 @@ __device__ __forceinline__ bool traverse_ww(const DevNode2* __restrict__ nodes, 
 // ("dynamic fetch", Aila & Laine 2009): the wave runs rounds of the speculative
 // while-while loop above, and before each round lanes whose ray has finished take new
 // rays from the queue once at least kRefill of them are idle, so a few long rays no
-// longer hold the whole wave (48 of 64 measured best on MI355X: a lower threshold
-// refills in small batches and mixes rays from distant queue slots, which costs
-// more cache locality than it saves). load(slot, o, d, tmin, tmax) reads a ray;
+// longer hold the whole wave (16 to 32 of 64 measured best on MI355X; 64, i.e. no
+// refill, is 18% slower). load(slot, o, d, tmin, tmax) reads a ray;
 // finish(slot, r, hit) consumes a result (hit: a hit was found).
-constexpr uint32_t kRefill = 48;
+constexpr uint32_t kRefill = 24;
+
+// Traversal stack of the single-level kernels: the top kRingEntries entries live in an
+// LDS ring (entry k in slot k % kRingEntries, lane-interleaved), older ones are moved
+// to the lane's area of a global spill buffer when the ring is full and brought back,
+// a ring at a time, when it runs empty. Deep traversals are rare, so the ring keeps
+// LDS small (4 KiB per wave) and occupancy bound by registers, not by the tree depth.
+constexpr int kRingEntries = 16;
+
+struct TwoTierStack {
+  uint32_t* ring;  // lane's first ring slot (slots 64 words apart)
+  uint32_t* spill; // lane's spill area (cap entries)
+  int sp;          // entries on the stack
+  int lo;          // entries [0, lo) are in the spill area, [lo, sp) in the ring
+};
+
+__device__ __forceinline__ void stack_push(TwoTierStack& s, uint32_t v) {
+  if (s.sp - s.lo == kRingEntries) {  // ring full: move its oldest entry out
+    s.spill[s.lo] = s.ring[(s.lo & (kRingEntries - 1)) * 64];
+    ++s.lo;
+  }
+  s.ring[(s.sp & (kRingEntries - 1)) * 64] = v;
+  ++s.sp;
+}
+
+__device__ __forceinline__ int32_t stack_pop(TwoTierStack& s) {
+  if (s.sp == 0) return kTravDone;
+  if (s.sp == s.lo) {  // ring empty: bring back up to a ring of spilled entries
+    const int n = s.lo < kRingEntries ? s.lo : kRingEntries;
+    for (int k = s.lo - n; k < s.lo; ++k) s.ring[(k & (kRingEntries - 1)) * 64] = s.spill[k];
+    s.lo -= n;
+  }
+  --s.sp;
+  return static_cast<int32_t>(s.ring[(s.sp & (kRingEntries - 1)) * 64]);
+}
 
 template <bool ANY, bool STATS, class Load, class Finish>
 __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in, uint32_t* stk, uint32_t* overflow,
@@ -383,13 +416,13 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
   r.b1 = r.b2 = 0.f;
   r.prim = kNoHit;
   r.inst = kNoInstance;
-  int sp = 0;
+  TwoTierStack st;
+  st.ring = stk;
+  st.spill = acc.spill + static_cast<size_t>(blockIdx.x * kBlock + threadIdx.x) * static_cast<uint32_t>(cap);
+  st.sp = 0;
+  st.lo = 0;
   int32_t node = kTravDone, leaf = 0;
-  auto pop = [&]() -> int32_t {
-    if (sp == 0) return kTravDone;
-    --sp;
-    return static_cast<int32_t>(stk[sp * 64]);
-  };
+  auto pop = [&]() -> int32_t { return stack_pop(st); };
   for (;;) {
     const uint64_t idle = __ballot(slot == kNoHit);
     const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
@@ -407,7 +440,8 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
           r.b1 = r.b2 = 0.f;
           r.prim = kNoHit;
           r.inst = kNoInstance;
-          sp = 0;
+          st.sp = 0;
+          st.lo = 0;
           node = 0;
           leaf = 0;
         }
@@ -433,9 +467,8 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
           nearc = c.y;
           farc = c.x;
         }
-        if (sp < cap) {
-          stk[sp * 64] = static_cast<uint32_t>(farc);
-          ++sp;
+        if (st.sp < cap) {
+          stack_push(st, static_cast<uint32_t>(farc));
         } else {
           atomicOr(overflow, 1u);
         }
@@ -645,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevAccel acc, RayQueue q, Hit
 template <bool STATS>
 __global__ __launch_bounds__(kBlock) void k_extend_sl(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
-  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane_id();
+  uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
@@ -672,7 +705,7 @@ template <bool STATS>
 __global__ __launch_bounds__(kBlock) void k_shadow_sl(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
                                                        uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
-  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane_id();
+  uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
@@ -1117,6 +1150,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const float* partial, uin
 }
 
 size_t stack_lds_bytes(const DevAccel& acc) { return static_cast<size_t>(acc.stack_cap) * 64u * 4u * kWavesPerBlock; }
+size_t ring_lds_bytes() { return static_cast<size_t>(kRingEntries) * 64u * 4u * kWavesPerBlock; }
 
 }  // namespace
 
@@ -1128,8 +1162,8 @@ int trace_blocks_per_cu(const DevAccel& acc) {
     ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false>, kBlock, lds);
     eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kBlock, lds);
   } else {
-    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, lds);
-    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, lds);
+    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, ring_lds_bytes());
+    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, ring_lds_bytes());
   }
   if (ea != hipSuccess) a = 4;
   if (eb != hipSuccess) b = 4;
@@ -1160,9 +1194,9 @@ void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t
                    hipStream_t stream) {
   if (!acc.two_level) {
     if (stats)
-      hipLaunchKernelGGL(k_extend_sl<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in, ctr);
+      hipLaunchKernelGGL(k_extend_sl<true>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, q, h, in, ctr);
     else
-      hipLaunchKernelGGL(k_extend_sl<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in,
+      hipLaunchKernelGGL(k_extend_sl<false>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, q, h, in,
                          ctr);
     return;
   }
@@ -1187,10 +1221,10 @@ void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn 
                    bool stats, hipStream_t stream) {
   if (!acc.two_level) {
     if (stats)
-      hipLaunchKernelGGL(k_shadow_sl<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance,
+      hipLaunchKernelGGL(k_shadow_sl<true>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, sq, radiance,
                          in, ctr);
     else
-      hipLaunchKernelGGL(k_shadow_sl<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance,
+      hipLaunchKernelGGL(k_shadow_sl<false>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, sq, radiance,
                          in, ctr);
     return;
   }
