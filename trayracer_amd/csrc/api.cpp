@@ -674,7 +674,8 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
     tlas_depth = tb.depth;
     n_nodes += tb.nodes.size();
   }
-  const int cap = depth + tlas_depth + 2;
+  // entries per lane: one per tree level, plus a leaf remainder and the exit marker per instance level
+  const int cap = depth + tlas_depth + (ctx->two_level ? 4 : 2);
   if (cap > 64) return fail(ctx, TRT_UNSUPPORTED, "BVH too deep for the LDS stack (" + std::to_string(cap) + ")");
   ctx->stack_cap = cap;
 

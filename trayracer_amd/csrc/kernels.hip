@@ -526,6 +526,194 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
   }
 }
 
+// Two-level counterpart of trace_queue (instanced scenes): the same speculative
+// while-while loop with lane refill and two-tier stack, walking the top-level tree over
+// instance boxes and, inside an instance, its mesh's tree with the ray in object space.
+// Entering an instance pushes what is left of the top-level leaf and an exit marker;
+// popping the marker returns the ray to world space. A lane with a parked (not yet
+// intersected) leaf never leaves its instance: it waits at the marker until the leaf
+// is done, since the leaf's triangles are tested in that instance's space.
+constexpr int32_t kTravExit = 0x7fffffff;                               // leave the instance (stack marker)
+constexpr int32_t kTravBlocked = static_cast<int32_t>(0x80000001u);     // waits at the marker with a parked leaf
+
+template <bool ANY, bool STATS, class Load, class Finish>
+__device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& in, uint32_t* stk, uint32_t* overflow,
+                                             TraceOut& stats, Load load, Finish finish) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_seg = seg_counts(in), home = wave_home();
+  const int cap = acc.stack_cap;
+  const DevNode2* __restrict__ blas = acc.nodes;
+  const DevNode2* __restrict__ tlas = acc.tlas;
+  const float4* __restrict__ tris = acc.tris;
+  uint64_t seg_done = seg_empty(n_seg);
+  bool drained = false;
+  uint32_t slot = kNoHit;
+  V3 ow = v3(0.f), dw = v3(0.f);
+  RayBox rbw{}, rb{};
+  RayTri rt{};
+  float tmin = 0.f;
+  TraceOut r;
+  r.t = 0.f;
+  r.b1 = r.b2 = 0.f;
+  r.prim = kNoHit;
+  r.inst = kNoInstance;
+  uint32_t cur_id = kNoInstance;
+  bool in_blas = false;
+  TwoTierStack st;
+  st.ring = stk;
+  st.spill = acc.spill + static_cast<size_t>(blockIdx.x * kBlock + threadIdx.x) * static_cast<uint32_t>(cap);
+  st.sp = 0;
+  st.lo = 0;
+  int32_t node = kTravDone, leaf = 0;
+  bool ok = true;  // false after a stack overflow (reported once per lane)
+  auto push = [&](uint32_t v) {
+    if (st.sp < cap) {
+      stack_push(st, v);
+    } else if (ok) {
+      ok = false;
+      atomicOr(overflow, 1u);
+    }
+  };
+  // Pops the next node; leaves the instance at its marker unless a leaf is parked.
+  auto pop = [&]() -> int32_t {
+    for (;;) {
+      const int32_t e = stack_pop(st);
+      if (e != kTravExit) return e;
+      if (leaf != 0) {
+        stack_push(st, static_cast<uint32_t>(kTravExit));
+        return kTravBlocked;
+      }
+      in_blas = false;
+      rb = rbw;
+    }
+  };
+  // Top-level leaf (instances [first, first + count)): enter the first instance.
+  auto enter = [&](int32_t code) -> int32_t {
+    const uint32_t c = ~static_cast<uint32_t>(code);
+    const uint32_t first = c >> 3, cnt = c & 7u;
+    if (cnt > 1u) push(~(((first + 1u) << 3) | (cnt - 1u)));  // the rest of the leaf
+    push(static_cast<uint32_t>(kTravExit));
+    const Instance& ins = acc.inst[first];
+    const V3 o = xform_point(ins.w2o, ow);
+    const V3 d = xform_vector(ins.w2o, dw);
+    rb = ray_box_setup(o, d);
+    rt = ray_tri_setup(o, d);
+    cur_id = ins.id;
+    in_blas = true;
+    if (STATS) stats.nodes++;
+    return static_cast<int32_t>(ins.blas_root);
+  };
+  // Resolves top-level leaves (entering instances) until node is an inner node, a
+  // mesh leaf, kTravBlocked or kTravDone.
+  auto settle = [&](int32_t n) -> int32_t {
+    while (!in_blas && n < 0 && n != kTravDone && n != kTravBlocked) n = enter(n);
+    return n;
+  };
+  for (;;) {
+    const uint64_t idle = __ballot(slot == kNoHit);
+    const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+    if (!drained && n_idle >= kRefill) {
+      uint32_t base = 0, got = 0;
+      if (seg_fetch_k(in, n_seg, home, seg_done, n_idle, base, got)) {
+        const uint32_t rank = prefix_count(idle);
+        if (slot == kNoHit && rank < got) {
+          slot = base + rank;
+          float tmax;
+          load(slot, ow, dw, tmin, tmax);
+          rbw = ray_box_setup(ow, dw);
+          rb = rbw;
+          in_blas = false;
+          r.t = tmax;
+          r.b1 = r.b2 = 0.f;
+          r.prim = kNoHit;
+          r.inst = kNoInstance;
+          st.sp = 0;
+          st.lo = 0;
+          node = 0;
+          leaf = 0;
+        }
+      } else {
+        drained = true;
+      }
+    }
+    if (!__any(slot != kNoHit)) break;
+    if (slot == kNoHit) continue;
+    bool hit = false;
+    // inner nodes of either level
+    while (node >= 0) {
+      float4 r0, r1, r2;
+      int4 c;
+      load_node(in_blas ? blas : tlas, node, r0, r1, r2, c);
+      if (STATS) stats.nodes++;
+      float t0, t1;
+      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+      if (h0 && h1) {
+        int32_t nearc = c.x, farc = c.y;
+        if (t1 < t0) {
+          nearc = c.y;
+          farc = c.x;
+        }
+        push(static_cast<uint32_t>(farc));
+        node = nearc;
+      } else if (h0) {
+        node = c.x;
+      } else if (h1) {
+        node = c.y;
+      } else {
+        node = pop();
+      }
+      node = settle(node);
+      if (in_blas && node < 0 && node != kTravDone && node != kTravBlocked && leaf == 0) {
+        leaf = node;
+        node = settle(pop());
+      }
+      if (__all(leaf != 0)) break;
+    }
+    // mesh leaves (in the current instance's space)
+    while (leaf != 0) {
+      const uint32_t code = ~static_cast<uint32_t>(leaf);
+      const uint32_t first = code >> 3, cnt = code & 7u;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const float4* tp = tris + static_cast<size_t>(first + k) * 3u;
+        const float4 a = tp[0], b = tp[1], cc = tp[2];
+        if (STATS) stats.tests++;
+        TriHit th;
+        if (intersect_triangle(rt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), tmin, r.t, th)) {
+          const uint32_t prim = __float_as_uint(a.w);
+          if (th.t < r.t || cur_id < r.inst || (cur_id == r.inst && prim < r.prim)) {
+            r.t = th.t;
+            r.b1 = th.b1;
+            r.b2 = th.b2;
+            r.prim = prim;
+            r.inst = cur_id;
+          }
+          if (ANY) {
+            hit = true;
+            break;
+          }
+        }
+      }
+      if (ANY && hit) break;
+      if (node == kTravBlocked) {
+        leaf = 0;
+        node = settle(pop());
+      } else if (in_blas && node < 0 && node != kTravDone) {
+        leaf = node;
+        node = settle(pop());
+      } else {
+        leaf = 0;
+      }
+    }
+    if (hit || (node == kTravDone && leaf == 0)) {
+      finish(slot, r, hit || r.prim != kNoHit);
+      slot = kNoHit;
+      node = kTravDone;
+      leaf = 0;
+    }
+  }
+}
+
 // Full trace (single or two-level). r.t must hold tmax on entry.
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trace_ray(const DevAccel& acc, V3 o, V3 d, float tmin, uint32_t* stk,
@@ -642,37 +830,6 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams rp, uint32_t n_p
   }
 }
 
-template <bool STATS>
-__global__ __launch_bounds__(kBlock) void k_extend(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
-  extern __shared__ uint32_t lds[];
-  const uint32_t lane = lane_id();
-  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
-  const uint32_t n_seg = seg_counts(in), home = wave_home();
-  uint64_t done = seg_empty(n_seg);
-  uint32_t slot0, n_valid;
-  while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
-    const uint32_t slot = slot0 + lane;
-    TraceOut r;
-    r.nodes = 0;
-    r.tests = 0;
-    if (lane < n_valid) {
-      const float4 a = q.o_tmin[slot];
-      const float4 b = q.d_tmax[slot];
-      r.t = b.w;
-      r.b1 = 0.f;
-      r.b2 = 0.f;
-      r.prim = kNoHit;
-      r.inst = kNoInstance;
-      trace_ray<false, STATS>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
-      h.thit[slot] = make_float4(r.t, r.b1, r.b2, __uint_as_float(r.prim));
-      if (acc.two_level) h.inst[slot] = r.inst;
-    }
-    if (STATS) {
-      wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 0]), r.nodes);
-      wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 2]), r.tests);
-    }
-  }
-}
 
 // Single-level scenes: Extend and Shadow through trace_queue.
 template <bool STATS>
@@ -736,44 +893,69 @@ __global__ __launch_bounds__(kBlock) void k_shadow_sl(DevAccel acc, ShadowQueue 
   }
 }
 
+// Two-level scenes: Extend and Shadow through trace_queue2.
 template <bool STATS>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
-                                                    uint32_t* ctr) {
+__global__ __launch_bounds__(kBlock) void k_extend_tl(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
-  const uint32_t lane = lane_id();
-  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
-  const uint32_t n_seg = seg_counts(in), home = wave_home();
-  uint64_t done = seg_empty(n_seg);
-  uint32_t slot0, n_valid;
-  while (seg_fetch(in, n_seg, home, done, slot0, n_valid)) {
-    const uint32_t slot = slot0 + lane;
-    TraceOut r;
-    r.nodes = 0;
-    r.tests = 0;
-    if (lane < n_valid) {
-      const float4 a = sq.o_tmin[slot];
-      const float4 b = sq.d_tmax[slot];
-      r.t = b.w;
-      r.prim = kNoHit;
-      r.inst = kNoInstance;
-      const bool occ =
-          trace_ray<true, STATS>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
-      if (!occ) {
-        const float4 c = sq.contrib[slot];
-        const uint32_t path = __float_as_uint(c.w);
-        float4 L = radiance[path];
-        L.x += c.x;
-        L.y += c.y;
-        L.z += c.z;
-        radiance[path] = L;
-      }
-    }
-    if (STATS) {
-      wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 4]), r.nodes);
-      wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 6]), r.tests);
-    }
+  uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
+  TraceOut st;
+  st.nodes = 0;
+  st.tests = 0;
+  trace_queue2<false, STATS>(
+      acc, in, stk, &ctr[kCtrOverflow], st,
+      [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
+        const float4 a = q.o_tmin[slot];
+        const float4 b = q.d_tmax[slot];
+        o = v3(a.x, a.y, a.z);
+        d = v3(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+      },
+      [&](uint32_t slot, const TraceOut& r, bool) {
+        h.thit[slot] = make_float4(r.t, r.b1, r.b2, __uint_as_float(r.prim));
+        h.inst[slot] = r.inst;
+      });
+  if (STATS) {
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 0]), st.nodes);
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 2]), st.tests);
   }
 }
+
+template <bool STATS>
+__global__ __launch_bounds__(kBlock) void k_shadow_tl(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
+                                                       uint32_t* ctr) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
+  TraceOut st;
+  st.nodes = 0;
+  st.tests = 0;
+  trace_queue2<true, STATS>(
+      acc, in, stk, &ctr[kCtrOverflow], st,
+      [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
+        const float4 a = sq.o_tmin[slot];
+        const float4 b = sq.d_tmax[slot];
+        o = v3(a.x, a.y, a.z);
+        d = v3(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+      },
+      [&](uint32_t slot, const TraceOut&, bool occluded) {
+        if (!occluded) {
+          const float4 c = sq.contrib[slot];
+          const uint32_t path = __float_as_uint(c.w);
+          float4 L = radiance[path];
+          L.x += c.x;
+          L.y += c.y;
+          L.z += c.z;
+          radiance[path] = L;
+        }
+      });
+  if (STATS) {
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 4]), st.nodes);
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 6]), st.tests);
+  }
+}
+
 
 // DEBUG: the render modes of render_modes.h instead of the path-tracing step.
 template <bool DEBUG>
@@ -1159,8 +1341,9 @@ int trace_blocks_per_cu(const DevAccel& acc) {
   const size_t lds = stack_lds_bytes(acc);
   hipError_t ea, eb;
   if (acc.two_level) {
-    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false>, kBlock, lds);
-    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kBlock, lds);
+    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_tl<false>, kBlock, ring_lds_bytes());
+    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_tl<false>, kBlock, ring_lds_bytes());
+    (void)lds;
   } else {
     ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, ring_lds_bytes());
     eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, ring_lds_bytes());
@@ -1201,9 +1384,9 @@ void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t
     return;
   }
   if (stats)
-    hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in, ctr);
+    hipLaunchKernelGGL(k_extend_tl<true>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, q, h, in, ctr);
   else
-    hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, q, h, in, ctr);
+    hipLaunchKernelGGL(k_extend_tl<false>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, q, h, in, ctr);
 }
 
 void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
@@ -1229,10 +1412,10 @@ void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn 
     return;
   }
   if (stats)
-    hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance, in,
+    hipLaunchKernelGGL(k_shadow_tl<true>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, sq, radiance, in,
                        ctr);
   else
-    hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, sq, radiance, in,
+    hipLaunchKernelGGL(k_shadow_tl<false>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, sq, radiance, in,
                        ctr);
 }
 
