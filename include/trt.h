@@ -365,6 +365,10 @@ trt_status trt_denoiser_backward(trt_context* ctx, const float* dout, uint32_t n
                                  float* grads);
 /* Post-activation output of convolution `index` (0..13) of the last forward. */
 trt_status trt_denoiser_activation(trt_context* ctx, uint32_t index, float* out, uint64_t n);
+/* Test hook: the inference path of trt_denoise on host features [N][H][W][16] ->
+ * radiance [N][H][W][4] = (expm1(max(feature c + output c, 0)) for c < 3, 1). */
+trt_status trt_denoiser_apply(trt_context* ctx, const float* features, uint32_t n, uint32_t height, uint32_t width,
+                              float* radiance);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,13 @@ def test_forward_and_backward_match_reference(rend):
         layer_err = max(layer_err, float(d.max()))
     print("per-layer forward max error:", layer_err)
     assert layer_err < 1e-2  # one bf16 ulp of the largest activation
+    # inference path (fused radiance epilogue, no fp32 output tensor) against the
+    # training path's output mapped back from log space on the host
+    rad = rend.denoiser_apply(feats)
+    want_rad = np.expm1(np.maximum(U.bf16(feats[..., :3]).astype(np.float64) + out[..., :3], 0.0))
+    assert np.allclose(rad[..., :3], want_rad, rtol=2e-6, atol=1e-7)
+    assert (rad[..., 3] == 1.0).all()
+    out = rend.denoiser_forward(feats)  # restore the training-path state for backward
     dout = rng.normal(size=(N, H, W, 32)).astype(np.float32)
     dout[..., 3:] = 0
     g = rend.denoiser_backward(dout)
@@ -71,6 +78,23 @@ def test_forward_and_backward_match_reference(rend):
               + " ".join(f"{x:.1e}" for x in per_tap))
         worst = max(worst, rw, rb)
     assert worst < 1e-2, worst
+
+
+def test_forward_with_odd_coarsest_level(rend):
+    """H = 24: the 8x-downsampled level has 3 rows, so the coarsest convolutions cut
+    their last 2-row pixel block (1080p has the same case at 135 rows)."""
+    from oracle import unet_ref as U
+    rng = np.random.default_rng(1)
+    params = U.init_params(11)
+    rend.denoiser_set_params(params)
+    feats = _rand_features(rng, 1, 24, 128)
+    out = rend.denoiser_forward(feats)
+    want = U.UnetRef(params, round_bf16=True).forward(feats)
+    assert np.abs(out - want).max() / np.abs(want).max() < 2e-2
+    assert np.sqrt(((out - want) ** 2).mean()) / np.sqrt((want ** 2).mean()) < 5e-3
+    rad = rend.denoiser_apply(feats)
+    want_rad = np.expm1(np.maximum(U.bf16(feats[..., :3]).astype(np.float64) + out[..., :3], 0.0))
+    assert np.allclose(rad[..., :3], want_rad, rtol=2e-6, atol=1e-7)
 
 
 def test_train_and_denoise_cornell(rend, tmp_path):

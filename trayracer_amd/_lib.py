@@ -189,6 +189,7 @@ PROTOTYPES = [
     ("trt_write_png", S, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, c_u8_p]),
     ("trt_denoiser_forward", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
     ("trt_denoiser_activation", S, [P, ctypes.c_uint32, c_float_p, ctypes.c_uint64]),
+    ("trt_denoiser_apply", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
     ("trt_denoiser_backward", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
 ]
 

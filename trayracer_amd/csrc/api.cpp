@@ -1167,8 +1167,9 @@ trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* st) {
   dn_features(ctx->color.as<float4>(), ctx->albedo.as<float4>(), ctx->normal.as<float4>(), ctx->dn_feat.as<uint16_t>(),
               P, ctx->stream);
   HIPC(hipEventRecord(ev[1], ctx->stream), "event");
-  HIPC(ctx->unet->forward(ctx->dn_feat.as<uint16_t>(), 1, int(ctx->height), int(ctx->width), ctx->stream), "forward");
-  dn_output(ctx->unet->output(), ctx->dn_feat.as<uint16_t>(), ctx->dn_denoised.as<float4>(), P, ctx->stream);
+  HIPC(ctx->unet->forward(ctx->dn_feat.as<uint16_t>(), 1, int(ctx->height), int(ctx->width), ctx->stream,
+                          ctx->dn_denoised.as<float4>()),
+       "forward");
   HIPC(hipEventRecord(ev[2], ctx->stream), "event");
   HIPC(hipStreamSynchronize(ctx->stream), "denoise");
   if (st) {
@@ -1226,6 +1227,29 @@ trt_status trt_denoiser_forward(trt_context* ctx, const float* features, uint32_
   HIPC(ctx->unet->forward(ctx->dn_io.as<uint16_t>(), int(N), int(H), int(W), ctx->stream), "forward");
   HIPC(hipMemcpyAsync(out, ctx->unet->output(), P * 32 * 4, hipMemcpyDeviceToHost, ctx->stream), "read");
   HIPC(hipStreamSynchronize(ctx->stream), "forward");
+  return TRT_OK;
+}
+
+// Inference path on host features (as trt_denoise runs it on the rendered buffers):
+// radiance [N][H][W][4] = (expm1(max(feature c + output c, 0)) for c < 3, 1).
+trt_status trt_denoiser_apply(trt_context* ctx, const float* features, uint32_t N, uint32_t H, uint32_t W,
+                              float* radiance) {
+  trt_status s = need_unet(ctx);
+  if (s != TRT_OK) return s;
+  if (!features || !radiance || N == 0 || !ctx->unet->shape_ok(int(H), int(W)))
+    return fail(ctx, TRT_INVALID_ARG, "need width % 128 == 0, height % 8 == 0");
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  const uint64_t P = uint64_t(N) * H * W;
+  std::vector<uint16_t> hb(P * kFeatures);
+  for (uint64_t i = 0; i < hb.size(); ++i) hb[i] = host_f2bf(features[i]);
+  DevBuf out;
+  HIPC(out.ensure(P * 16), "alloc");
+  HIPC(ctx->dn_io.ensure(P * kFeatures * 2), "alloc");
+  HIPC(hipMemcpyAsync(ctx->dn_io.p, hb.data(), hb.size() * 2, hipMemcpyHostToDevice, ctx->stream), "upload");
+  HIPC(ctx->unet->forward(ctx->dn_io.as<uint16_t>(), int(N), int(H), int(W), ctx->stream, out.as<float4>()),
+       "forward");
+  HIPC(hipMemcpyAsync(radiance, out.p, P * 16, hipMemcpyDeviceToHost, ctx->stream), "read");
+  HIPC(hipStreamSynchronize(ctx->stream), "apply");
   return TRT_OK;
 }
 

@@ -114,9 +114,6 @@ hipError_t Unet::ensure(int N, int H, int W) {
   if ((e = alloc(P1 * 32 * 2, reinterpret_cast<void**>(&p0_))) != hipSuccess) return e;
   if ((e = alloc(P2 * 64 * 2, reinterpret_cast<void**>(&p1_))) != hipSuccess) return e;
   if ((e = alloc(P3 * 96 * 2, reinterpret_cast<void**>(&p2_))) != hipSuccess) return e;
-  if ((e = alloc(P2 * 224 * 2, reinterpret_cast<void**>(&u2_))) != hipSuccess) return e;
-  if ((e = alloc(P1 * 160 * 2, reinterpret_cast<void**>(&u1_))) != hipSuccess) return e;
-  if ((e = alloc(P0 * 96 * 2, reinterpret_cast<void**>(&u0_))) != hipSuccess) return e;
   if ((e = alloc(P0 * 32 * 4, reinterpret_cast<void**>(&out_))) != hipSuccess) return e;
   // backward scratch: the largest per-pixel gradient tensor is 96 ch at full resolution
   const size_t gmax = P0 * 96;
@@ -153,76 +150,108 @@ size_t Unet::activation_elems(int i) const {
   return lvl[i] * size_t(convs_[size_t(i)].cout);
 }
 
-hipError_t Unet::forward(const uint16_t* x, int N, int H, int W, hipStream_t s) {
+hipError_t Unet::forward(const uint16_t* x, int N, int H, int W, hipStream_t s, float4* radiance) {
   if (!w_ || !shape_ok(H, W)) return hipErrorInvalidValue;
   hipError_t e = ensure(N, H, W);
   if (e != hipSuccess) return e;
   x_ = x;
+  have_output_ = false;
   const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
-  auto conv = [&](int i, const uint16_t* in, void* out, int h, int w, bool f32) {
+  bool ok = true;
+  // conv i on [upsample(coarse) | in] at h x w; pooled: fused 2x2 pooling of the output
+  auto conv = [&](int i, const uint16_t* coarse, int c1, const uint16_t* in, int h, int w, uint16_t* pooled) {
     const Conv& c = convs_[i];
-    dn_conv3x3(in, wb_ + c.woff, w_ + c.boff, out, N, h, w, c.cin, c.cout, c.relu, f32, s);
+    DnConv d;
+    d.x = in;
+    d.coarse = coarse;
+    d.c1 = c1;
+    d.cx = c.cin - c1;
+    d.w = wb_ + c.woff;
+    d.bias = w_ + c.boff;
+    d.y = act_[i];
+    d.cout = c.cout;
+    d.pooled = pooled;
+    d.N = N;
+    d.H = h;
+    d.W = w;
+    d.epi = pooled ? kEpiReluPool : kEpiRelu;
+    ok = ok && dn_conv(d, s);
   };
-  conv(E0A, x, act_[E0A], H, W, false);
-  conv(E0B, act_[E0A], act_[E0B], H, W, false);
-  dn_pool(act_[E0B], p0_, N, H, W, 32, s);
-  conv(E1A, p0_, act_[E1A], H1, W1, false);
-  conv(E1B, act_[E1A], act_[E1B], H1, W1, false);
-  dn_pool(act_[E1B], p1_, N, H1, W1, 64, s);
-  conv(E2A, p1_, act_[E2A], H2, W2, false);
-  conv(E2B, act_[E2A], act_[E2B], H2, W2, false);
-  dn_pool(act_[E2B], p2_, N, H2, W2, 96, s);
-  conv(E3A, p2_, act_[E3A], H3, W3, false);
-  conv(E3B, act_[E3A], act_[E3B], H3, W3, false);
-  dn_upcat(act_[E3B], act_[E2B], u2_, N, H2, W2, 128, 96, s);
-  conv(D2A, u2_, act_[D2A], H2, W2, false);
-  conv(D2B, act_[D2A], act_[D2B], H2, W2, false);
-  dn_upcat(act_[D2B], act_[E1B], u1_, N, H1, W1, 96, 64, s);
-  conv(D1A, u1_, act_[D1A], H1, W1, false);
-  conv(D1B, act_[D1A], act_[D1B], H1, W1, false);
-  dn_upcat(act_[D1B], act_[E0B], u0_, N, H, W, 64, 32, s);
-  conv(D0A, u0_, act_[D0A], H, W, false);
-  conv(D0B, act_[D0A], act_[D0B], H, W, false);
-  conv(OUT, act_[D0B], out_, H, W, true);
+  conv(E0A, nullptr, 0, x, H, W, nullptr);
+  conv(E0B, nullptr, 0, act_[E0A], H, W, p0_);
+  conv(E1A, nullptr, 0, p0_, H1, W1, nullptr);
+  conv(E1B, nullptr, 0, act_[E1A], H1, W1, p1_);
+  conv(E2A, nullptr, 0, p1_, H2, W2, nullptr);
+  conv(E2B, nullptr, 0, act_[E2A], H2, W2, p2_);
+  conv(E3A, nullptr, 0, p2_, H3, W3, nullptr);
+  conv(E3B, nullptr, 0, act_[E3A], H3, W3, nullptr);
+  conv(D2A, act_[E3B], 128, act_[E2B], H2, W2, nullptr);
+  conv(D2B, nullptr, 0, act_[D2A], H2, W2, nullptr);
+  conv(D1A, act_[D2B], 96, act_[E1B], H1, W1, nullptr);
+  conv(D1B, nullptr, 0, act_[D1A], H1, W1, nullptr);
+  conv(D0A, act_[D1B], 64, act_[E0B], H, W, nullptr);
+  conv(D0B, nullptr, 0, act_[D0A], H, W, nullptr);
+  const Conv& c = convs_[OUT];
+  DnConv d;
+  d.x = act_[D0B];
+  d.cx = c.cin;
+  d.w = wb_ + c.woff;
+  d.bias = w_ + c.boff;
+  d.cout = c.cout;
+  d.N = N;
+  d.H = H;
+  d.W = W;
+  if (radiance) {
+    d.feat = x;
+    d.radiance = radiance;
+    d.epi = kEpiRadiance;
+  } else {
+    d.y = out_;
+    d.epi = kEpiF32;
+  }
+  ok = ok && dn_conv(d, s);
+  if (!ok) return hipErrorInvalidValue;
+  have_output_ = radiance == nullptr;
   return hipGetLastError();
 }
 
 hipError_t Unet::backward(const float* dout, int N, int H, int W, hipStream_t s) {
-  if (N != N_ || H != H_ || W != W_ || !x_) return hipErrorInvalidValue;
+  if (N != N_ || H != H_ || W != W_ || !x_ || !have_output_) return hipErrorInvalidValue;
   const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
   // one conv layer: gradient in (fp32, post-activation) -> dW, db, gradient out (fp32)
-  auto layer = [&](int i, const uint16_t* in, int h, int w, const float* gin, float* gout) {
+  auto layer = [&](int i, const uint16_t* coarse, int c1, const uint16_t* in, int h, int w, const float* gin,
+                   float* gout) {
     const Conv& c = convs_[i];
     const size_t P = size_t(N) * h * w;
     if (c.relu)
       dn_relu_bwd(gin, act_[i], dz_, P * c.cout, s);
     else
       dn_to_bf16(gin, dz_, P * c.cout, s);
-    dn_wgrad(in, dz_, wpart_, g_ + c.woff, N, h, w, c.cin, c.cout, kWgradTiles, s);
+    dn_wgrad(in, coarse, c1, dz_, wpart_, g_ + c.woff, N, h, w, c.cin, c.cout, kWgradTiles, s);
     dn_bias_grad(dz_, P, c.cout, bpart_, g_ + c.boff, s);
     if (gout) dn_conv3x3(dz_, wd_ + c.woff, nullptr, gout, N, h, w, c.cout, c.cin, false, true, s);
   };
-  layer(OUT, act_[D0B], H, W, dout, gA_);
-  layer(D0B, act_[D0A], H, W, gA_, gB_);
-  layer(D0A, u0_, H, W, gB_, gA_);                          // gA = d u0 (96 ch @H)
+  layer(OUT, nullptr, 0, act_[D0B], H, W, dout, gA_);
+  layer(D0B, nullptr, 0, act_[D0A], H, W, gA_, gB_);
+  layer(D0A, act_[D1B], 64, act_[E0B], H, W, gB_, gA_);  // gA = d u0 (96 ch @H)
   dn_upcat_bwd(gA_, gB_, gs0_, N, H, W, 64, 32, false, s);  // gB = d d1b (64 @H1), gs0 = d e0b (skip part)
-  layer(D1B, act_[D1A], H1, W1, gB_, gA_);
-  layer(D1A, u1_, H1, W1, gA_, gB_);                           // gB = d u1 (160 @H1)
+  layer(D1B, nullptr, 0, act_[D1A], H1, W1, gB_, gA_);
+  layer(D1A, act_[D2B], 96, act_[E1B], H1, W1, gA_, gB_);  // gB = d u1 (160 @H1)
   dn_upcat_bwd(gB_, gA_, gs1_, N, H1, W1, 96, 64, false, s);  // gA = d d2b (96 @H2), gs1 = d e1b
-  layer(D2B, act_[D2A], H2, W2, gA_, gB_);
-  layer(D2A, u2_, H2, W2, gB_, gA_);                           // gA = d u2 (224 @H2)
+  layer(D2B, nullptr, 0, act_[D2A], H2, W2, gA_, gB_);
+  layer(D2A, act_[E3B], 128, act_[E2B], H2, W2, gB_, gA_);  // gA = d u2 (224 @H2)
   dn_upcat_bwd(gA_, gB_, gs2_, N, H2, W2, 128, 96, false, s);  // gB = d e3b (128 @H3), gs2 = d e2b
-  layer(E3B, act_[E3A], H3, W3, gB_, gA_);
-  layer(E3A, p2_, H3, W3, gA_, gB_);                           // gB = d p2 (96 @H3)
-  dn_pool_bwd(gB_, gs2_, N, H2, W2, 96, true, s);              // gs2 += pool backward
-  layer(E2B, act_[E2A], H2, W2, gs2_, gA_);
-  layer(E2A, p1_, H2, W2, gA_, gB_);                           // gB = d p1 (64 @H2)
+  layer(E3B, nullptr, 0, act_[E3A], H3, W3, gB_, gA_);
+  layer(E3A, nullptr, 0, p2_, H3, W3, gA_, gB_);  // gB = d p2 (96 @H3)
+  dn_pool_bwd(gB_, gs2_, N, H2, W2, 96, true, s);  // gs2 += pool backward
+  layer(E2B, nullptr, 0, act_[E2A], H2, W2, gs2_, gA_);
+  layer(E2A, nullptr, 0, p1_, H2, W2, gA_, gB_);  // gB = d p1 (64 @H2)
   dn_pool_bwd(gB_, gs1_, N, H1, W1, 64, true, s);
-  layer(E1B, act_[E1A], H1, W1, gs1_, gA_);
-  layer(E1A, p0_, H1, W1, gA_, gB_);                           // gB = d p0 (32 @H1)
+  layer(E1B, nullptr, 0, act_[E1A], H1, W1, gs1_, gA_);
+  layer(E1A, nullptr, 0, p0_, H1, W1, gA_, gB_);  // gB = d p0 (32 @H1)
   dn_pool_bwd(gB_, gs0_, N, H, W, 32, true, s);
-  layer(E0B, act_[E0A], H, W, gs0_, gA_);
-  layer(E0A, x_, H, W, gA_, nullptr);  // the input gradient of the network is not needed
+  layer(E0B, nullptr, 0, act_[E0A], H, W, gs0_, gA_);
+  layer(E0A, nullptr, 0, x_, H, W, gA_, nullptr);  // the input gradient of the network is not needed
   return hipGetLastError();
 }
 

@@ -13,19 +13,39 @@ namespace trt {
 constexpr int kBiasBlocks = 256;
 constexpr int kFeatures = 16;  // input channels (10 used: radiance, albedo, normal, depth)
 
-// conv launchers: false when the shape is unsupported (Cin % 16, Cout % 32, W % 16)
+// One 3x3 convolution (denoiser_kernels.hip). Input [upsample(coarse) | x] (coarse
+// nullable, c1 = 0), weights [cout][9][c1 + cx], bias nullable. Epilogues:
+//   kEpiRelu      y = bf16(relu(conv + bias))
+//   kEpiF32       y = fp32(conv + bias)
+//   kEpiReluPool  as kEpiRelu, plus pooled = bf16(2x2 mean of the bf16 y)
+//   kEpiRadiance  radiance[p] = (expm1(max(feat[p][c] + conv + bias, 0)) for c < 3, 1)
+enum DnEpi { kEpiRelu = 0, kEpiF32 = 1, kEpiReluPool = 2, kEpiRadiance = 3 };
+struct DnConv {
+  const uint16_t* x = nullptr;
+  const uint16_t* coarse = nullptr;
+  int c1 = 0, cx = 0;
+  const uint16_t* w = nullptr;
+  const float* bias = nullptr;
+  void* y = nullptr;
+  int cout = 0;
+  uint16_t* pooled = nullptr;
+  const uint16_t* feat = nullptr;
+  float4* radiance = nullptr;
+  int N = 0, H = 0, W = 0;
+  int epi = kEpiRelu;
+};
+// false when the shape is unsupported (channels % 32 or Cin = 16, Cout % 32, W % 16,
+// odd H with pooling or a coarse input)
+bool dn_conv(const DnConv& c, hipStream_t s);
 bool dn_conv3x3(const uint16_t* X, const uint16_t* Wt, const float* bias, void* Y, int N, int H, int W, int Cin,
                 int Cout, bool relu, bool out_f32, hipStream_t s);
 int dn_wgrad_chunks(int N, int H, int W, int tiles_per_chunk);
-void dn_wgrad(const uint16_t* X, const uint16_t* dZ, float* partial, float* dW, int N, int H, int W, int Cin, int Cout,
-              int tiles_per_chunk, hipStream_t s);
+void dn_wgrad(const uint16_t* X, const uint16_t* coarse, int c1, const uint16_t* dZ, float* partial, float* dW, int N,
+              int H, int W, int Cin, int Cout, int tiles_per_chunk, hipStream_t s);
 void dn_bias_grad(const uint16_t* dZ, size_t P, int C, float* partial, float* db, hipStream_t s);
 void dn_relu_bwd(const float* dY, const uint16_t* Y, uint16_t* dZ, size_t n, hipStream_t s);
 void dn_to_bf16(const float* a, uint16_t* b, size_t n, hipStream_t s);
-void dn_pool(const uint16_t* X, uint16_t* Y, int N, int H, int W, int C, hipStream_t s);
 void dn_pool_bwd(const float* dY, float* dX, int N, int H, int W, int C, bool accumulate, hipStream_t s);
-void dn_upcat(const uint16_t* coarse, const uint16_t* skip, uint16_t* out, int N, int H, int W, int C1, int C2,
-              hipStream_t s);
 void dn_upcat_bwd(const float* dout, float* dcoarse, float* dskip, int N, int H, int W, int C1, int C2,
                   bool accumulate_skip, hipStream_t s);
 void dn_features(const float4* color, const float4* albedo, const float4* normal, uint16_t* out, size_t P,
@@ -34,7 +54,6 @@ void dn_target(const float4* color, float* out, size_t P, hipStream_t s);
 void dn_gather_crop(const uint16_t* feat, const float* tgt, int Hs, int Ws, const int4* crops, int B, int S,
                     uint16_t* fo, float* to, hipStream_t s);
 int dn_loss(const float* Y, const uint16_t* X, const float* T, size_t P, float* dY, float* partial, hipStream_t s);
-void dn_output(const float* Y, const uint16_t* X, float4* out, size_t P, hipStream_t s);
 void dn_adam(float* w, float* m, float* v, const float* g, uint16_t* wb, const uint8_t* mask, size_t n, float lr,
              float b1, float b2, float c1, float c2, float eps, hipStream_t s);
 void dn_flip_weights(const uint16_t* w, uint16_t* wd, int cin, int cout, hipStream_t s);
@@ -58,8 +77,11 @@ class Unet {
   hipError_t init(uint32_t seed, hipStream_t s);
   hipError_t set_params(const float* host, hipStream_t s);
   hipError_t get_params(float* host, hipStream_t s);
-  // forward on features [N][H][W][16] (device); output fp32 [N][H][W][32]
-  hipError_t forward(const uint16_t* x, int N, int H, int W, hipStream_t s);
+  // forward on features [N][H][W][16] (device). With radiance == nullptr: output fp32
+  // [N][H][W][32] in output(), activations kept for backward(). Otherwise the last layer
+  // writes the denoised radiance (x + y mapped back from log space) and output() and
+  // backward() are not available until the next such forward.
+  hipError_t forward(const uint16_t* x, int N, int H, int W, hipStream_t s, float4* radiance = nullptr);
   const float* output() const { return out_; }
   // training step on a batch; returns the loss through *loss (synchronises)
   hipError_t train_step(const uint16_t* x, const float* target, int N, int H, int W, float lr, hipStream_t s,
@@ -81,10 +103,11 @@ class Unet {
   size_t n_params_ = 0;
   float *w_ = nullptr, *m_ = nullptr, *v_ = nullptr, *g_ = nullptr;  // fp32 master, Adam moments, grads
   uint16_t *wb_ = nullptr, *wd_ = nullptr;                              // bf16 weights, flipped for dgrad
-  // activations (bf16) per conv output, plus pooled / concatenated inputs
+  // activations (bf16) per conv output, plus the pooled encoder outputs
   std::vector<uint16_t*> act_;
-  uint16_t *p0_ = nullptr, *p1_ = nullptr, *p2_ = nullptr, *u2_ = nullptr, *u1_ = nullptr, *u0_ = nullptr;
+  uint16_t *p0_ = nullptr, *p1_ = nullptr, *p2_ = nullptr;
   float* out_ = nullptr;
+  bool have_output_ = false;  // out_ holds the last forward's output
   const uint16_t* x_ = nullptr;  // input of the last forward (read by the first layer's wgrad)
   // backward scratch
   float *gA_ = nullptr, *gB_ = nullptr, *gs0_ = nullptr, *gs1_ = nullptr, *gs2_ = nullptr, *dout_ = nullptr;

@@ -17,6 +17,7 @@ namespace dn {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -29,87 +30,333 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 constexpr int kTileH = 8, kTileW = 16;          // output pixels per workgroup
 constexpr int kPatchH = kTileH + 2, kPatchW = kTileW + 2;
 constexpr int kPatch = kPatchH * kPatchW;        // 180 staged input positions
-constexpr int kChunk = 32;                       // input channels staged per step
-constexpr int kPosStride = kChunk + 8;           // bf16 per LDS position (padding vs bank conflicts)
 
 // ------------------------------------------------------------------------------------
-// 3x3 'same' convolution. X [N][H][W][Cin] bf16, Wt [Cout][9][Cin] bf16, bias [Cout]
-// fp32 (nullable). Output bf16 (optionally ReLU) or fp32. Cin % 16 == 0, Cout = 32*NB.
-// Grid: (W/16, ceil(H/8), N); block 256 = 4 waves, wave w owns tile rows 2w, 2w+1.
-template <int NB, bool RELU, bool OUT_F32>
-__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
-                                                  const float* __restrict__ bias, void* __restrict__ Y, int H, int W,
-                                                  int Cin) {
-  __shared__ uint16_t patch[kPatch * kPosStride];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x0 = blockIdx.x * kTileW, y0 = blockIdx.y * kTileH, n = blockIdx.z;
-  const int Cout = NB * 32;
-  const size_t img = static_cast<size_t>(n) * H * W;
-  f32x16 acc[NB];
-  for (int b = 0; b < NB; ++b)
-    for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
-  // this lane's A row: pixel p = lane % 32 of the wave's 2x16 block
-  const int p = lane & 31;
-  const int prow = 2 * wave + (p >> 4), pcol = p & 15;
-  const int khalf = (lane >> 5) * 8;  // k offset of this lane inside a 16-wide step
+// 3x3 'same' convolution as an implicit GEMM (M = pixels, N = output channels,
+// K = 9 taps x Cin), input and weights staged through LDS per 32-channel chunk.
+//
+// Input: `x` [N][H][W][cx] bf16, optionally preceded (in channel order) by `coarse`
+// [N][H/2][W/2][c1] bf16 read with nearest upsampling, so a decoder convolution reads
+// [upsample(coarse) | skip] without materialising the concatenation. Cin = c1 + cx;
+// c1, cx multiples of 32, or Cin = 16 without coarse. Weights [Cout][9][Cin] bf16.
+//
+// Workgroup = 4 waves over a (8 MB) x 16 pixel tile and 32 NB output channels (one
+// channel group); wave w owns MB m-blocks of 2 x 16 pixels (rows 2(w MB + m) + 0..1)
+// and all NB n-blocks, so every A fragment feeds NB MFMAs and every B fragment MB.
+// The next chunk's global loads are issued before the current chunk's MFMAs.
+struct ConvArgs {
+  const uint16_t* x;
+  const uint16_t* coarse;
+  int c1, cx;
+  const uint16_t* w;
+  const float* bias;  // nullable
+  void* y;            // bf16 or fp32 [N][H][W][cout]; unused by kEpiRadiance
+  int cout;
+  uint16_t* pooled;     // kEpiReluPool: 2x2 average of the bf16 output [N][H/2][W/2][cout]
+  const uint16_t* feat; // kEpiRadiance: input features [N][H][W][16]
+  float4* radiance;     // kEpiRadiance: expm1(max(feat[0..2] + y[0..2], 0)), alpha 1
+  int N, H, W;
+  int groups;  // cout / (32 NB)
+};
 
-  for (int c0 = 0; c0 < Cin; c0 += kChunk) {
-    const int cc = Cin - c0 < kChunk ? Cin - c0 : kChunk;  // 16 or 32
-    __syncthreads();
-    // stage the (10 x 18) x cc patch, 8 channels (16 bytes) per thread-iteration
-    const int groups = cc / 8;
-    for (int i = threadIdx.x; i < kPatch * groups; i += 256) {
-      const int pos = i / groups, g = i - pos * groups;
-      const int py = pos / kPatchW, px = pos - py * kPatchW;
-      const int yy = y0 + py - 1, xx = x0 + px - 1;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-        v = *reinterpret_cast<const uint4*>(X + (img + static_cast<size_t>(yy) * W + xx) * Cin + c0 + g * 8);
-      *reinterpret_cast<uint4*>(patch + pos * kPosStride + g * 8) = v;
-    }
-    __syncthreads();
-    for (int tap = 0; tap < 9; ++tap) {
-      const int ky = tap / 3, kx = tap - ky * 3;
-      const uint16_t* arow = patch + ((prow + ky) * kPatchW + (pcol + kx)) * kPosStride;
-      for (int ks = 0; ks < cc; ks += 16) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + ks + khalf);
-        for (int b = 0; b < NB; ++b) {
-          const int co = b * 32 + (lane & 31);
-          const bf16x8 bv =
-              *reinterpret_cast<const bf16x8*>(Wt + (static_cast<size_t>(co) * 9 + tap) * Cin + c0 + ks + khalf);
-          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[b], 0, 0, 0);
-        }
+constexpr int kPosStride = 40;           // bf16 per staged position (32 + 8 padding)
+constexpr int kWStride = 9 * 32 + 8;     // bf16 per staged output channel
+
+template <int MB, int NB, int EPI>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+  constexpr int TH = 8 * MB, PH = TH + 2, PW = kTileW + 2, NPOS = PH * PW;
+  constexpr int PLOADS = (NPOS * 4 + 255) / 256;
+  constexpr int WLOADS = (32 * NB * 9 * 4 + 255) / 256;
+  constexpr int OS = 32 * NB + 8;                                 // bf16 per pixel in the output staging
+  constexpr int OWAVE = (EPI == kEpiF32 ? 8 : MB * 32 * OS);      // output staging per wave (bf16)
+  // one spare 16-byte slot at the end of patch and wl takes the stores of idle threads
+  __shared__ __align__(16) uint16_t patch[NPOS * kPosStride + 8];
+  __shared__ __align__(16) uint16_t wl[32 * NB * kWStride + 8];
+  __shared__ __align__(16) uint16_t ob[4 * OWAVE];
+  __shared__ __align__(16) float bl[32 * NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int H = a.H, W = a.W, Cin = a.c1 + a.cx, Cout = a.cout;
+  const int gs = Cin == 16 ? 1 : 2;  // log2 of 16-byte groups per staged position
+  const int cc = 8 << gs;            // channels per chunk
+  const int nchunks = Cin / cc;
+  const int tiles_x = W / kTileW, tiles_y = (H + TH - 1) / TH;
+  const int total = tiles_x * tiles_y * a.N * a.groups;
+  // Persistent workgroups. Workgroups are dealt round-robin over the 8 XCDs, so XCD
+  // x = blockIdx % 8 gets one contiguous range of items (neighbouring tiles share
+  // their halo rows in that XCD's L2); its workgroups stride through the range.
+  // Item order: channel group fastest, then tile column, tile row, image.
+  const int per = static_cast<int>(gridDim.x) >> 3, xcd = static_cast<int>(blockIdx.x) & 7;
+  const int q = total >> 3, rem = total & 7;
+  const int begin = xcd * q + (xcd < rem ? xcd : rem);
+  const int end = begin + q + (xcd < rem ? 1 : 0);
+  int item = begin + (static_cast<int>(blockIdx.x) >> 3), chunk = 0;
+  if (item >= end) return;
+  struct Tile {
+    int x0, y0, co0, n;
+  };
+  auto decode = [&](int it) __attribute__((always_inline)) {
+    Tile t;
+    const int g = it % a.groups;
+    it /= a.groups;
+    const int tx = it % tiles_x;
+    it /= tiles_x;
+    t.x0 = tx * kTileW;
+    t.y0 = (it % tiles_y) * TH;
+    t.n = it / tiles_y;
+    t.co0 = g * 32 * NB;
+    return t;
+  };
+
+  f32x16 acc[MB][NB];
+  for (int m = 0; m < MB; ++m)
+    for (int j = 0; j < NB; ++j)
+      for (int r = 0; r < 16; ++r) acc[m][j][r] = 0.f;
+
+  // Staging: global -> registers (issued one step ahead) -> LDS. Thread tid stages the
+  // same patch positions and weight slots on every step; their tile-relative offsets
+  // are computed once here. Every thread issues the same loads and LDS stores on every
+  // step (halo positions outside the image load a valid address and store zero; idle
+  // threads store to the spare slot), so no load or store sits in a branch and the wait
+  // for the staged registers is a single wait on every path.
+  int pyx[PLOADS];  // patch position: row | column << 8 | channel group << 16; -1 idle
+  int plds[PLOADS], wsrc[WLOADS], wlds[WLOADS];
+#pragma unroll
+  for (int k = 0; k < PLOADS; ++k) {
+    const int i = tid + k * 256;
+    const int pos = i >> gs, gi = i & ((1 << gs) - 1), py = pos / PW, px = pos - py * PW;
+    const bool ok = i < (NPOS << gs);
+    pyx[k] = ok ? py | (px << 8) | (gi << 16) : -1;
+    plds[k] = ok ? pos * kPosStride + gi * 8 : NPOS * kPosStride;
+  }
+#pragma unroll
+  for (int k = 0; k < WLOADS; ++k) {
+    const int i = tid + k * 256;
+    const bool ok = i < (32 * NB * 9 << gs);
+    const int qq = ok ? i >> gs : 0, gi = i & ((1 << gs) - 1), co = qq / 9, tap = qq - co * 9;
+    wsrc[k] = (co * 9 + tap) * Cin + gi * 8;
+    wlds[k] = ok ? co * kWStride + tap * 32 + gi * 8 : 32 * NB * kWStride;
+  }
+  u32x4 pr[PLOADS], wr[WLOADS];
+  uint32_t pvalid = 0;  // bit k: pr[k] holds image data (else zero padding)
+  float br = 0.f;
+  uint2 fr_next = make_uint2(0u, 0u), fr_cur = make_uint2(0u, 0u);  // kEpiRadiance features
+  int prev_key = -1;
+  bool wload = false;
+  auto load = [&](int it, int ch) __attribute__((always_inline)) {
+    const Tile t = decode(it);
+    const int c0 = ch * cc;
+    pvalid = 0;
+    if (c0 < a.c1) {  // chunk of the nearest-upsampled coarse input
+      const uint16_t* base = a.coarse + static_cast<size_t>(t.n) * (H >> 1) * (W >> 1) * a.c1 + c0;
+#pragma unroll
+      for (int k = 0; k < PLOADS; ++k) {
+        const int yy = t.y0 + (pyx[k] & 0xff) - 1, xx = t.x0 + ((pyx[k] >> 8) & 0xff) - 1;
+        const bool ok = pyx[k] >= 0 && static_cast<unsigned>(yy) < static_cast<unsigned>(H) &&
+                        static_cast<unsigned>(xx) < static_cast<unsigned>(W);
+        const int off = ok ? ((yy >> 1) * (W >> 1) + (xx >> 1)) * a.c1 + ((pyx[k] >> 16) << 3) : 0;
+        pr[k] = *reinterpret_cast<const u32x4*>(base + off);
+        pvalid |= (ok ? 1u : 0u) << k;
+      }
+    } else {
+      const uint16_t* base = a.x + static_cast<size_t>(t.n) * H * W * a.cx + (c0 - a.c1);
+#pragma unroll
+      for (int k = 0; k < PLOADS; ++k) {
+        const int yy = t.y0 + (pyx[k] & 0xff) - 1, xx = t.x0 + ((pyx[k] >> 8) & 0xff) - 1;
+        const bool ok = pyx[k] >= 0 && static_cast<unsigned>(yy) < static_cast<unsigned>(H) &&
+                        static_cast<unsigned>(xx) < static_cast<unsigned>(W);
+        const int off = ok ? (yy * W + xx) * a.cx + ((pyx[k] >> 16) << 3) : 0;
+        pr[k] = *reinterpret_cast<const u32x4*>(base + off);
+        pvalid |= (ok ? 1u : 0u) << k;
       }
     }
-  }
-  // epilogue
-  for (int b = 0; b < NB; ++b) {
-    const int co = b * 32 + (lane & 31);
-    const float bb = bias ? bias[co] : 0.f;
-    for (int r = 0; r < 16; ++r) {
-      const int pi = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      const int yy = y0 + 2 * wave + (pi >> 4), xx = x0 + (pi & 15);
-      if (yy >= H) continue;
-      float v = acc[b][r] + bb;
-      if (RELU) v = v > 0.f ? v : 0.f;
-      const size_t o = (img + static_cast<size_t>(yy) * W + xx) * Cout + co;
-      if (OUT_F32)
-        static_cast<float*>(Y)[o] = v;
-      else
-        static_cast<uint16_t*>(Y)[o] = f2bf(v);
+    const int key = (t.co0 / 32) * nchunks + ch;
+    wload = key != prev_key;
+    prev_key = key;
+    if (wload) {
+      const uint16_t* wb = a.w + static_cast<size_t>(t.co0) * 9 * Cin + c0;
+#pragma unroll
+      for (int k = 0; k < WLOADS; ++k) wr[k] = *reinterpret_cast<const u32x4*>(wb + wsrc[k]);
+      br = a.bias ? a.bias[t.co0 + (tid & (32 * NB - 1))] : 0.f;
     }
+    if (EPI == kEpiRadiance) {  // single-chunk layer: features 0..3 of this lane's pixel
+      const int pi = lane & 31, yy = t.y0 + 2 * (wave * MB + (lane >> 5) % MB) + (pi >> 4);
+      const size_t p = (static_cast<size_t>(t.n) * H + (yy < H ? yy : 0)) * W + t.x0 + (pi & 15);
+      fr_next = *reinterpret_cast<const uint2*>(a.feat + p * 16);
+    }
+  };
+  auto store = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < PLOADS; ++k)
+      *reinterpret_cast<u32x4*>(patch + plds[k]) = (pvalid >> k) & 1u ? pr[k] : u32x4{0u, 0u, 0u, 0u};
+    if (wload) {
+#pragma unroll
+      for (int k = 0; k < WLOADS; ++k) *reinterpret_cast<u32x4*>(wl + wlds[k]) = wr[k];
+      if (tid < 32 * NB) bl[tid] = br;
+    }
+  };
+  // Output of a finished tile, staged per wave in LDS by the epilogue ([pixel][channel]
+  // bf16) and written one step later with 16-byte stores, so that the stores are issued
+  // before the next tile's loads (one in-order counter tracks both) and never delay the
+  // wait for them. The 2x2 pooling reads the staged bf16 values back.
+  uint16_t* obw = ob + wave * OWAVE;
+  auto flush = [&](int it) __attribute__((always_inline)) {
+    const Tile t = decode(it);
+    if (EPI == kEpiRadiance) {
+      if (lane < 32 * MB) {
+        const int pi = lane & 31, yy = t.y0 + 2 * (wave * MB + (lane >> 5)) + (pi >> 4);
+        const float4 y = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(obw) + lane * 4);
+        if (yy < H) {
+          float4 o;
+          o.x = expm1f(fmaxf(bf2f(static_cast<uint16_t>(fr_cur.x & 0xffffu)) + y.x, 0.f));
+          o.y = expm1f(fmaxf(bf2f(static_cast<uint16_t>(fr_cur.x >> 16)) + y.y, 0.f));
+          o.z = expm1f(fmaxf(bf2f(static_cast<uint16_t>(fr_cur.y & 0xffffu)) + y.z, 0.f));
+          o.w = 1.f;
+          a.radiance[(static_cast<size_t>(t.n) * H + yy) * W + t.x0 + (pi & 15)] = o;
+        }
+      }
+      return;
+    }
+    constexpr int PARTS = 4 * NB;  // 16-byte parts per pixel
+    uint16_t* yb = static_cast<uint16_t*>(a.y) + static_cast<size_t>(t.n) * H * W * Cout + t.co0;
+#pragma unroll
+    for (int k = 0; k < (MB * 32 * PARTS) / 64; ++k) {
+      const int c = lane + 64 * k, px = c / PARTS, part = c - px * PARTS;
+      const int pi = px & 31, yy = t.y0 + 2 * (wave * MB + (px >> 5)) + (pi >> 4);
+      const u32x4 v = *reinterpret_cast<const u32x4*>(obw + px * OS + part * 8);
+      if (yy < H) *reinterpret_cast<u32x4*>(yb + (yy * W + t.x0 + (pi & 15)) * Cout + part * 8) = v;
+    }
+    if (EPI == kEpiReluPool) {  // H even: pooled rows are whole
+      uint16_t* pb = a.pooled + static_cast<size_t>(t.n) * (H >> 1) * (W >> 1) * Cout + t.co0;
+      for (int c = lane; c < MB * 8 * PARTS; c += 64) {
+        const int pq = c / PARTS, part = c - pq * PARTS, m = pq >> 3, px = 2 * (pq & 7);
+        const int yy = t.y0 + 2 * (wave * MB + m);
+        const uint16_t* s = obw + (m * 32 + px) * OS + part * 8;
+        const u32x4 v00 = *reinterpret_cast<const u32x4*>(s), v01 = *reinterpret_cast<const u32x4*>(s + OS);
+        const u32x4 v10 = *reinterpret_cast<const u32x4*>(s + 16 * OS);
+        const u32x4 v11 = *reinterpret_cast<const u32x4*>(s + 17 * OS);
+        u32x4 o;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          // two channels per dword; top pair + bottom pair as in a stand-alone pooling
+          const float lo = (__uint_as_float(v00[d] << 16) + __uint_as_float(v01[d] << 16)) +
+                           (__uint_as_float(v10[d] << 16) + __uint_as_float(v11[d] << 16));
+          const float hi = (__uint_as_float(v00[d] & 0xffff0000u) + __uint_as_float(v01[d] & 0xffff0000u)) +
+                           (__uint_as_float(v10[d] & 0xffff0000u) + __uint_as_float(v11[d] & 0xffff0000u));
+          o[d] = static_cast<uint32_t>(f2bf(0.25f * lo)) | (static_cast<uint32_t>(f2bf(0.25f * hi)) << 16);
+        }
+        if (yy < H)
+          *reinterpret_cast<u32x4*>(pb + ((yy >> 1) * (W >> 1) + (t.x0 >> 1) + (px >> 1)) * Cout + part * 8) = o;
+      }
+    }
+  };
+
+  const int l32 = lane & 31, khalf = (lane >> 5) * 8;
+  bool pend = false;
+  int pend_item = 0;
+  load(item, 0);
+  for (;;) {
+    const int cur = item, cur_chunk = chunk;
+    if (++chunk == nchunks) {
+      chunk = 0;
+      item += per;
+    }
+    const bool more = item < end;
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (pend) {
+      flush(pend_item);
+      pend = false;
+    }
+    if (EPI == kEpiRadiance) fr_cur = fr_next;
+    if (more) load(item, chunk);
+    // MFMA with the weights as A (rows = output channels) and the patch as B (columns =
+    // pixels): accumulator register r of lane l then holds pixel l % 32 of the m-block
+    // and channel 8 (r / 4) + 4 (l / 32) + r % 4 of the n-block, i.e. four runs of four
+    // consecutive channels of one pixel.
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+#pragma unroll
+      for (int ks = 0; ks < 32; ks += 16) {
+        if (ks >= cc) break;
+        bf16x8 av[MB], bv[NB];
+#pragma unroll
+        for (int m = 0; m < MB; ++m) {
+          const int row = 2 * (wave * MB + m) + (l32 >> 4) + ky, col = (l32 & 15) + kx;
+          av[m] = *reinterpret_cast<const bf16x8*>(patch + (row * PW + col) * kPosStride + ks + khalf);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          bv[j] = *reinterpret_cast<const bf16x8*>(wl + (j * 32 + l32) * kWStride + tap * 32 + ks + khalf);
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            acc[m][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[j], av[m], acc[m][j], 0, 0, 0);
+      }
+    }
+    if (cur_chunk == nchunks - 1) {
+      const Tile t = decode(cur);
+      const int pi = l32, h4 = 4 * (lane >> 5);
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        const int yy = t.y0 + 2 * (wave * MB + m) + (pi >> 4);
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int qd = 0; qd < 4; ++qd) {
+            const int col = j * 32 + 8 * qd + h4;  // first of four channels
+            const float4 bb = *reinterpret_cast<const float4*>(bl + col);
+            float v0 = acc[m][j][4 * qd] + bb.x, v1 = acc[m][j][4 * qd + 1] + bb.y;
+            float v2 = acc[m][j][4 * qd + 2] + bb.z, v3 = acc[m][j][4 * qd + 3] + bb.w;
+            if (EPI == kEpiRelu || EPI == kEpiReluPool) {
+              v0 = fmaxf(v0, 0.f);
+              v1 = fmaxf(v1, 0.f);
+              v2 = fmaxf(v2, 0.f);
+              v3 = fmaxf(v3, 0.f);
+            }
+            if (EPI == kEpiF32) {
+              if (yy < H)
+                *reinterpret_cast<float4*>(static_cast<float*>(a.y) +
+                                           ((static_cast<size_t>(t.n) * H + yy) * W + t.x0 + (pi & 15)) * Cout +
+                                           t.co0 + col) = make_float4(v0, v1, v2, v3);
+            } else if (EPI == kEpiRadiance) {
+              if (j == 0 && qd == 0 && h4 == 0)
+                *reinterpret_cast<float4*>(reinterpret_cast<float*>(obw) + (m * 32 + pi) * 4) =
+                    make_float4(v0, v1, v2, v3);
+            } else {
+              uint2 o;
+              o.x = static_cast<uint32_t>(f2bf(v0)) | (static_cast<uint32_t>(f2bf(v1)) << 16);
+              o.y = static_cast<uint32_t>(f2bf(v2)) | (static_cast<uint32_t>(f2bf(v3)) << 16);
+              *reinterpret_cast<uint2*>(obw + (m * 32 + pi) * OS + col) = o;
+            }
+            acc[m][j][4 * qd] = 0.f;
+            acc[m][j][4 * qd + 1] = 0.f;
+            acc[m][j][4 * qd + 2] = 0.f;
+            acc[m][j][4 * qd + 3] = 0.f;
+          }
+      }
+      if (EPI != kEpiF32) {
+        pend = true;
+        pend_item = cur;
+      }
+    }
+    if (!more) break;
   }
+  if (pend) flush(pend_item);
 }
 
 // ------------------------------------------------------------------------------------
-// Weight gradient: dW[co][tap][ci] = sum_p dZ[p][co] X[p + s(tap)][ci], as an MFMA GEMM
+// Weight gradient: dW[co][tap][ci] = sum_p dZ[p][co] X[p + s(tap)][ci] (X optionally
+// [upsample(coarse) | X] as in the forward convolution), as an MFMA GEMM
 // with M = co (32-block), N = ci (32-block), K = pixels. Each workgroup covers `tiles`
 // consecutive 8x16 pixel tiles of one image row band and writes its partial sums to
 // partial[chunk][co][tap][ci]; a fixed-order reduction over chunks follows.
 // Grid: (n_chunks, Cout/32 * ceil(Cin/32)); wave w handles taps w, w+4, w+8.
-__global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ X, const uint16_t* __restrict__ dZ,
-                                                float* __restrict__ partial, int N, int H, int W, int Cin, int Cout,
-                                                int tiles_per_chunk) {
+__global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ X, const uint16_t* __restrict__ coarse,
+                                                int c1, const uint16_t* __restrict__ dZ, float* __restrict__ partial,
+                                                int N, int H, int W, int Cin, int Cout, int tiles_per_chunk) {
   __shared__ uint16_t xs[32 * (kPatch + 8)];   // [ci][pos]
   __shared__ uint16_t zs[32 * (kTileH * kTileW + 8)];  // [co][pix]
   constexpr int XS = kPatch + 8, ZS = kTileH * kTileW + 8;
@@ -131,13 +378,17 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ X, c
     const int ty = tr / tiles_x, tx = tr - ty * tiles_x;
     const int y0 = ty * kTileH, x0 = tx * kTileW;
     const size_t img = static_cast<size_t>(n) * H * W;
+    const size_t img_c = static_cast<size_t>(n) * (H >> 1) * (W >> 1);
+    const int cx = Cin - c1;
     __syncthreads();
     for (int i = threadIdx.x; i < 32 * kPatch; i += 256) {
       const int ci = i / kPatch, pos = i - ci * kPatch;
       const int py = pos / kPatchW, px = pos - py * kPatchW;
       const int yy = y0 + py - 1, xx = x0 + px - 1, c = ib * 32 + ci;
       uint16_t v = 0;
-      if (c < Cin && yy >= 0 && yy < H && xx >= 0 && xx < W) v = X[(img + static_cast<size_t>(yy) * W + xx) * Cin + c];
+      if (c < Cin && yy >= 0 && yy < H && xx >= 0 && xx < W)
+        v = c < c1 ? coarse[(img_c + static_cast<size_t>(yy >> 1) * (W >> 1) + (xx >> 1)) * c1 + c]
+                   : X[(img + static_cast<size_t>(yy) * W + xx) * cx + (c - c1)];
       xs[ci * XS + pos] = v;
     }
     for (int i = threadIdx.x; i < 32 * kTileH * kTileW; i += 256) {
@@ -211,24 +462,6 @@ __global__ void k_to_bf16(const float* __restrict__ a, uint16_t* __restrict__ b,
   if (i < n) b[i] = f2bf(a[i]);
 }
 
-// 2x2 average pooling, bf16 -> bf16 (H, W even)
-__global__ void k_pool(const uint16_t* __restrict__ X, uint16_t* __restrict__ Y, int N, int H, int W, int C) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int Ho = H / 2, Wo = W / 2;
-  const size_t total = static_cast<size_t>(N) * Ho * Wo * C;
-  if (i >= total) return;
-  const int c = static_cast<int>(i % C);
-  size_t r = i / C;
-  const int x = static_cast<int>(r % Wo);
-  r /= Wo;
-  const int y = static_cast<int>(r % Ho);
-  const int n = static_cast<int>(r / Ho);
-  const size_t b = ((static_cast<size_t>(n) * H + 2 * y) * W + 2 * x) * C + c;
-  const float s = (bf2f(X[b]) + bf2f(X[b + C])) + (bf2f(X[b + static_cast<size_t>(W) * C]) +
-                                                    bf2f(X[b + static_cast<size_t>(W) * C + C]));
-  Y[i] = f2bf(0.25f * s);
-}
-
 // dX (fine, fp32) (+)= dY(coarse)/4 broadcast; accumulate selects += vs =
 __global__ void k_pool_bwd(const float* __restrict__ dY, float* __restrict__ dX, int N, int H, int W, int C,
                            int accumulate) {
@@ -243,23 +476,6 @@ __global__ void k_pool_bwd(const float* __restrict__ dY, float* __restrict__ dX,
   const int n = static_cast<int>(r / H);
   const float g = 0.25f * dY[((static_cast<size_t>(n) * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
   dX[i] = accumulate ? dX[i] + g : g;
-}
-
-// out[n][y][x] = [ coarse[n][y/2][x/2][0:C1] | skip[n][y][x][0:C2] ]
-__global__ void k_upcat(const uint16_t* __restrict__ coarse, const uint16_t* __restrict__ skip,
-                        uint16_t* __restrict__ out, int N, int H, int W, int C1, int C2) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int C = C1 + C2;
-  const size_t total = static_cast<size_t>(N) * H * W * C;
-  if (i >= total) return;
-  const int c = static_cast<int>(i % C);
-  size_t r = i / C;
-  const int x = static_cast<int>(r % W);
-  r /= W;
-  const int y = static_cast<int>(r % H);
-  const int n = static_cast<int>(r / H);
-  out[i] = c < C1 ? coarse[((static_cast<size_t>(n) * (H / 2) + y / 2) * (W / 2) + x / 2) * C1 + c]
-                  : skip[((static_cast<size_t>(n) * H + y) * W + x) * C2 + (c - C1)];
 }
 
 // coarse gradient: d_coarse[n][y][x][c] = sum of the 2x2 fine gradients of channel c < C1
@@ -378,16 +594,6 @@ __global__ void k_loss(const float* __restrict__ Y, const uint16_t* __restrict__
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// network output (log-space correction of the input) -> radiance
-__global__ void k_output(const float* __restrict__ Y, const uint16_t* __restrict__ X, float4* __restrict__ out,
-                         size_t P) {
-  const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  float v[3];
-  for (int c = 0; c < 3; ++c) v[c] = expm1f(fmaxf(bf2f(X[p * 16 + c]) + Y[p * 32 + c], 0.f));
-  out[p] = make_float4(v[0], v[1], v[2], 1.f);
-}
-
 // Adam (Kingma & Ba 2015) on fp32 master weights; bf16 copy refreshed; `mask` zeroes
 // updates of padding entries (nullable)
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
@@ -424,32 +630,101 @@ inline unsigned blocks_for(size_t n, unsigned b = 256) { return static_cast<unsi
 
 using namespace dn;
 
-#define TRT_CONV_CASE(NBV)                                                                                       \
-  case NBV:                                                                                                      \
-    if (out_f32)                                                                                                 \
-      hipLaunchKernelGGL((k_conv3x3<NBV, false, true>), grid, dim3(256), 0, s, X, Wt, bias, Y, H, W, Cin);     \
-    else if (relu)                                                                                               \
-      hipLaunchKernelGGL((k_conv3x3<NBV, true, false>), grid, dim3(256), 0, s, X, Wt, bias, Y, H, W, Cin);     \
-    else                                                                                                         \
-      hipLaunchKernelGGL((k_conv3x3<NBV, false, false>), grid, dim3(256), 0, s, X, Wt, bias, Y, H, W, Cin);    \
-    break;
+// Persistent grid: as many workgroups as fit on the device at once (occupancy of the
+// instantiation x CUs), a multiple of 8 (one share per XCD) whose per-XCD share is a
+// multiple of the channel-group count, so a workgroup keeps one group's weights.
+template <typename K>
+static unsigned persistent_blocks(K kernel, long items, int groups) {
+  static thread_local int cus = 0, dev = -1;
+  int d = 0;
+  (void)hipGetDevice(&d);
+  if (d != dev) {
+    dev = d;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || cus <= 0) cus = 256;
+  }
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, 256, 0) != hipSuccess || occ <= 0) occ = 1;
+  long per = long(cus) * occ / 8;
+  per = per / groups * groups;
+  if (per < groups) per = groups;
+  const long need = (items + 7) / 8;
+  if (per > need) per = (need + groups - 1) / groups * groups;
+  return static_cast<unsigned>(per * 8);
+}
+
+template <int MB, int NB, int EPI>
+static void launch_conv_epi(const ConvArgs& a, long items, hipStream_t s) {
+  const unsigned blocks = persistent_blocks(k_conv<MB, NB, EPI>, items, a.groups);
+  hipLaunchKernelGGL((k_conv<MB, NB, EPI>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <int MB, int NB>
+static void launch_conv(const ConvArgs& a, int epi, long items, hipStream_t s) {
+  switch (epi) {
+    case kEpiRelu: launch_conv_epi<MB, NB, kEpiRelu>(a, items, s); break;
+    case kEpiF32: launch_conv_epi<MB, NB, kEpiF32>(a, items, s); break;
+    case kEpiReluPool: launch_conv_epi<MB, NB, kEpiReluPool>(a, items, s); break;
+    default: launch_conv_epi<MB, NB, kEpiRadiance>(a, items, s); break;
+  }
+}
+
+// Tile shape per layer: 16-row tiles (MB = 2) unless that leaves < 1024 tiles, then
+// 8-row tiles with 64-channel groups (NB = 2) when Cout is a multiple of 64.
+bool dn_conv(const DnConv& c, hipStream_t s) {
+  const int Cin = c.c1 + c.cx;
+  const bool shape_ok = (Cin == 16 && c.c1 == 0) || (c.c1 % 32 == 0 && c.cx % 32 == 0 && Cin > 0);
+  if (!shape_ok || c.cout % 32 || c.W % kTileW || c.H <= 0 || c.N <= 0) return false;
+  if ((c.epi == kEpiReluPool || c.c1) && c.H % 2) return false;
+  if (c.epi == kEpiRadiance && c.cout < 4) return false;
+  if (c.c1 && !c.coarse) return false;
+  // element offsets inside one image are 32-bit
+  const long widest = c.cout > c.cx ? c.cout : c.cx;
+  if (long(c.H) * c.W * widest >= (1L << 31)) return false;
+  if (c.epi == kEpiRadiance && Cin > 32) return false;  // feature prefetch assumes one chunk
+  const long tiles16 = long(c.W / kTileW) * ((c.H + 15) / 16) * c.N;
+  const int MB = tiles16 >= 1024 ? 2 : 1;
+  const long tiles = long(c.W / kTileW) * ((c.H + 8 * MB - 1) / (8 * MB)) * c.N;
+  const int NB = MB == 1 && c.cout % 64 == 0 ? 2 : 1;  // MB = NB = 2 would need > 100 KB of LDS
+  ConvArgs a;
+  a.x = c.x;
+  a.coarse = c.coarse;
+  a.c1 = c.c1;
+  a.cx = c.cx;
+  a.w = c.w;
+  a.bias = c.bias;
+  a.y = c.y;
+  a.cout = c.cout;
+  a.pooled = c.pooled;
+  a.feat = c.feat;
+  a.radiance = c.radiance;
+  a.N = c.N;
+  a.H = c.H;
+  a.W = c.W;
+  a.groups = c.cout / (32 * NB);
+  const long items = tiles * a.groups;
+  if (MB == 2) {
+    launch_conv<2, 1>(a, c.epi, items, s);
+  } else {
+    if (NB == 2) launch_conv<1, 2>(a, c.epi, items, s); else launch_conv<1, 1>(a, c.epi, items, s);
+  }
+  return true;
+}
 
 bool dn_conv3x3(const uint16_t* X, const uint16_t* Wt, const float* bias, void* Y, int N, int H, int W, int Cin,
                 int Cout, bool relu, bool out_f32, hipStream_t s) {
-  if (Cin % 16 || Cout % 32 || W % kTileW || Cout > 224) return false;
-  const dim3 grid(W / kTileW, (H + kTileH - 1) / kTileH, N);
-  switch (Cout / 32) {
-    TRT_CONV_CASE(1)
-    TRT_CONV_CASE(2)
-    TRT_CONV_CASE(3)
-    TRT_CONV_CASE(4)
-    TRT_CONV_CASE(5)
-    TRT_CONV_CASE(6)
-    TRT_CONV_CASE(7)
-    default:
-      return false;
-  }
-  return true;
+  if (relu == out_f32) return false;  // bf16 + ReLU or fp32 linear
+  DnConv c;
+  c.x = X;
+  c.cx = Cin;
+  c.w = Wt;
+  c.bias = bias;
+  c.y = Y;
+  c.cout = Cout;
+  c.N = N;
+  c.H = H;
+  c.W = W;
+  c.epi = relu ? kEpiRelu : kEpiF32;
+  return dn_conv(c, s);
 }
 
 int dn_wgrad_chunks(int N, int H, int W, int tiles_per_chunk) {
@@ -457,11 +732,12 @@ int dn_wgrad_chunks(int N, int H, int W, int tiles_per_chunk) {
   return (tiles + tiles_per_chunk - 1) / tiles_per_chunk;
 }
 
-void dn_wgrad(const uint16_t* X, const uint16_t* dZ, float* partial, float* dW, int N, int H, int W, int Cin, int Cout,
-              int tiles_per_chunk, hipStream_t s) {
+void dn_wgrad(const uint16_t* X, const uint16_t* coarse, int c1, const uint16_t* dZ, float* partial, float* dW, int N,
+              int H, int W, int Cin, int Cout, int tiles_per_chunk, hipStream_t s) {
   const int chunks = dn_wgrad_chunks(N, H, W, tiles_per_chunk);
   const dim3 grid(chunks, (Cout / 32) * ((Cin + 31) / 32));
-  hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, s, X, dZ, partial, N, H, W, Cin, Cout, tiles_per_chunk);
+  hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, s, X, coarse, c1, dZ, partial, N, H, W, Cin, Cout,
+                     tiles_per_chunk);
   const size_t n = static_cast<size_t>(Cout) * 9 * Cin;
   hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks_for(n)), dim3(256), 0, s, partial, chunks, n, dW);
 }
@@ -478,18 +754,9 @@ void dn_relu_bwd(const float* dY, const uint16_t* Y, uint16_t* dZ, size_t n, hip
 void dn_to_bf16(const float* a, uint16_t* b, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(n)), dim3(256), 0, s, a, b, n);
 }
-void dn_pool(const uint16_t* X, uint16_t* Y, int N, int H, int W, int C, hipStream_t s) {
-  hipLaunchKernelGGL(k_pool, dim3(blocks_for(static_cast<size_t>(N) * (H / 2) * (W / 2) * C)), dim3(256), 0, s, X, Y,
-                     N, H, W, C);
-}
 void dn_pool_bwd(const float* dY, float* dX, int N, int H, int W, int C, bool accumulate, hipStream_t s) {
   hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for(static_cast<size_t>(N) * H * W * C)), dim3(256), 0, s, dY, dX, N, H,
                      W, C, accumulate ? 1 : 0);
-}
-void dn_upcat(const uint16_t* coarse, const uint16_t* skip, uint16_t* out, int N, int H, int W, int C1, int C2,
-              hipStream_t s) {
-  hipLaunchKernelGGL(k_upcat, dim3(blocks_for(static_cast<size_t>(N) * H * W * (C1 + C2))), dim3(256), 0, s, coarse,
-                     skip, out, N, H, W, C1, C2);
 }
 void dn_upcat_bwd(const float* dout, float* dcoarse, float* dskip, int N, int H, int W, int C1, int C2,
                   bool accumulate_skip, hipStream_t s) {
@@ -515,9 +782,6 @@ int dn_loss(const float* Y, const uint16_t* X, const float* T, size_t P, float* 
   const unsigned nb = blocks_for(P);
   hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, Y, X, T, P, dY, partial);
   return static_cast<int>(nb);
-}
-void dn_output(const float* Y, const uint16_t* X, float4* out, size_t P, hipStream_t s) {
-  hipLaunchKernelGGL(k_output, dim3(blocks_for(P)), dim3(256), 0, s, Y, X, out, P);
 }
 void dn_adam(float* w, float* m, float* v, const float* g, uint16_t* wb, const uint8_t* mask, size_t n, float lr,
              float b1, float b2, float c1, float c2, float eps, hipStream_t s) {

@@ -314,6 +314,15 @@ class Renderer:
         self._check(self.lib.trt_denoiser_forward(self.ctx, _fp(f), N, H, W, _fp(out)))
         return out
 
+    def denoiser_apply(self, features):
+        """Inference path of denoise() on given features -> radiance [N][H][W][4]."""
+        f = np.ascontiguousarray(features, np.float32)
+        N, H, W, C = f.shape
+        assert C == 16
+        out = np.empty((N, H, W, 4), np.float32)
+        self._check(self.lib.trt_denoiser_apply(self.ctx, _fp(f), N, H, W, _fp(out)))
+        return out
+
     def denoiser_activation(self, index, shape):
         out = np.empty(shape, np.float32)
         self._check(self.lib.trt_denoiser_activation(self.ctx, index, _fp(out), out.size))
