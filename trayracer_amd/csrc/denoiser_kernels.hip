@@ -59,151 +59,204 @@ struct ConvArgs {
   int groups;  // cout / (32 NB)
 };
 
+typedef unsigned v4u32_t __attribute__((ext_vector_type(4)));
+// third dword of a raw buffer resource on gfx9 parts (32-bit data format, no swizzle)
+constexpr int kBufFlags = 0x00020000;
+
+// two floats -> packed bf16 (round to nearest even), low half first
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  const f32x2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+// LDS layouts, in 16-byte slots. A 128-bit LDS read serves each half-wave (32 lanes) at
+// once and is conflict-free when their 32 slots are distinct modulo 32. A-fragment
+// reads take 16 consecutive patch positions of two consecutive rows: 5 slots per
+// position (odd, so 16 positions are distinct mod 32) and a row pitch of 112 slots
+// (= 16 mod 32, so the second row fills the other 16 residues). B-fragment reads take
+// 32 consecutive output channels 37 slots apart (odd).
 constexpr int kPosStride = 40;           // bf16 per staged position (32 + 8 padding)
+constexpr int kRowPitch = 112 * 8;       // bf16 per staged patch row (18 positions + padding)
 constexpr int kWStride = 9 * 32 + 8;     // bf16 per staged output channel
 
-template <int MB, int NB, int EPI>
+template <int MB, int NB, int EPI, bool SC, bool C16>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   constexpr int TH = 8 * MB, PH = TH + 2, PW = kTileW + 2, NPOS = PH * PW;
   constexpr int PLOADS = (NPOS * 4 + 255) / 256;
   constexpr int WLOADS = (32 * NB * 9 * 4 + 255) / 256;
   constexpr int OS = 32 * NB + 8;                                 // bf16 per pixel in the output staging
   constexpr int OWAVE = (EPI == kEpiF32 ? 8 : MB * 32 * OS);      // output staging per wave (bf16)
+  constexpr int PARTS = 4 * NB;                                   // 16-byte parts per staged pixel
+  constexpr int FLUSH = (MB * 32 * PARTS) / 64;                   // 16-byte stores per lane per tile
+  constexpr int kOOB = 0x7fffffff;                                // buffer offset past every range
   // one spare 16-byte slot at the end of patch and wl takes the stores of idle threads
-  __shared__ __align__(16) uint16_t patch[NPOS * kPosStride + 8];
+  __shared__ __align__(16) uint16_t patch[PH * kRowPitch + 8];
   __shared__ __align__(16) uint16_t wl[32 * NB * kWStride + 8];
   __shared__ __align__(16) uint16_t ob[4 * OWAVE];
   __shared__ __align__(16) float bl[32 * NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int H = a.H, W = a.W, Cin = a.c1 + a.cx, Cout = a.cout;
-  const int gs = Cin == 16 ? 1 : 2;  // log2 of 16-byte groups per staged position
-  const int cc = 8 << gs;            // channels per chunk
-  const int nchunks = Cin / cc;
+  constexpr int gs = C16 ? 1 : 2;    // log2 of 16-byte groups per staged position
+  constexpr int cc = 8 << gs;        // channels per chunk (Cin = 16 is one 16-channel chunk)
+  constexpr int KS = cc / 16;        // MFMA k-steps per tap
+  constexpr int NK = 9 * KS;         // MFMA k-steps per chunk
+  const int nchunks = Cin / cc;      // 1 when SC (single chunk)
   const int tiles_x = W / kTileW, tiles_y = (H + TH - 1) / TH;
   const int total = tiles_x * tiles_y * a.N * a.groups;
   // Persistent workgroups. Workgroups are dealt round-robin over the 8 XCDs, so XCD
   // x = blockIdx % 8 gets one contiguous range of items (neighbouring tiles share
-  // their halo rows in that XCD's L2); its workgroups stride through the range.
+  // their halo rows in that XCD's L2); its workgroups stride through the range by
+  // `per`, a multiple of the group count, so one workgroup keeps one channel group.
   // Item order: channel group fastest, then tile column, tile row, image.
   const int per = static_cast<int>(gridDim.x) >> 3, xcd = static_cast<int>(blockIdx.x) & 7;
   const int q = total >> 3, rem = total & 7;
   const int begin = xcd * q + (xcd < rem ? xcd : rem);
   const int end = begin + q + (xcd < rem ? 1 : 0);
-  int item = begin + (static_cast<int>(blockIdx.x) >> 3), chunk = 0;
-  if (item >= end) return;
-  struct Tile {
+  const int first = begin + (static_cast<int>(blockIdx.x) >> 3);
+  if (first >= end) return;
+  struct Step {
+    int it, ch;  // item, input-channel chunk
     int x0, y0, co0, n;
   };
-  auto decode = [&](int it) __attribute__((always_inline)) {
-    Tile t;
+  auto set_item = [&](Step& s, int it) __attribute__((always_inline)) {
+    s.it = it;
+    it = it < end ? it : end - 1;  // past the end: stage the last item again (discarded)
     const int g = it % a.groups;
     it /= a.groups;
     const int tx = it % tiles_x;
     it /= tiles_x;
-    t.x0 = tx * kTileW;
-    t.y0 = (it % tiles_y) * TH;
-    t.n = it / tiles_y;
-    t.co0 = g * 32 * NB;
-    return t;
+    s.x0 = tx * kTileW;
+    s.y0 = (it % tiles_y) * TH;
+    s.n = it / tiles_y;
+    s.co0 = g * 32 * NB;
+  };
+  auto advance = [&](Step s) __attribute__((always_inline)) {
+    if (++s.ch == nchunks) {
+      s.ch = 0;
+      set_item(s, s.it + per);
+    }
+    return s;
   };
 
   f32x16 acc[MB][NB];
+#pragma unroll
   for (int m = 0; m < MB; ++m)
-    for (int j = 0; j < NB; ++j)
-      for (int r = 0; r < 16; ++r) acc[m][j][r] = 0.f;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[m][j] = f32x16{};
 
-  // Staging: global -> registers (issued one step ahead) -> LDS. Thread tid stages the
-  // same patch positions and weight slots on every step; their tile-relative offsets
-  // are computed once here. Every thread issues the same loads and LDS stores on every
-  // step (halo positions outside the image load a valid address and store zero; idle
-  // threads store to the spare slot), so no load or store sits in a branch and the wait
-  // for the staged registers is a single wait on every path.
-  int pyx[PLOADS];  // patch position: row | column << 8 | channel group << 16; -1 idle
-  int plds[PLOADS], wsrc[WLOADS], wlds[WLOADS];
+  // Staging: global -> registers -> LDS, the patch two steps ahead (two register sets),
+  // the weights (reloaded per step unless SC) and features one step ahead, through
+  // buffer loads: a buffer resource per image, per-thread byte offsets fixed for the
+  // kernel (relative to the tile's corner), and out-of-range offsets reading zero (the
+  // padding of the 'same' convolution). Every thread issues the same loads on every
+  // step and none sits in a branch, so the wait before each LDS store counts exactly
+  // the loads issued after it. Idle threads store to the spare slot.
+  int pyx[PLOADS];  // patch position: row | column << 8
+  int prel[PLOADS], crel[PLOADS], plds[PLOADS];
+  uint32_t pidle = 0;
 #pragma unroll
   for (int k = 0; k < PLOADS; ++k) {
     const int i = tid + k * 256;
     const int pos = i >> gs, gi = i & ((1 << gs) - 1), py = pos / PW, px = pos - py * PW;
     const bool ok = i < (NPOS << gs);
-    pyx[k] = ok ? py | (px << 8) | (gi << 16) : -1;
-    plds[k] = ok ? pos * kPosStride + gi * 8 : NPOS * kPosStride;
+    pidle |= (ok ? 1u : 0u) << k;
+    pyx[k] = py | (px << 8);
+    prel[k] = ((py * W + px) * a.cx + gi * 8) * 2;
+    crel[k] = (((py + 1) >> 1) * (W >> 1) + ((px + 1) >> 1)) * a.c1 * 2 + (gi * 8) * 2;
+    plds[k] = ok ? py * kRowPitch + px * kPosStride + gi * 8 : PH * kRowPitch;
   }
+  u32x4 prA[PLOADS], prB[PLOADS], wr[WLOADS];
+  float br = 0.f;
+  uint2 fr_next = make_uint2(0u, 0u), fr_cur = make_uint2(0u, 0u);  // kEpiRadiance features
+  const uint32_t img_bytes = static_cast<uint32_t>(H) * W * a.cx * 2;
+  const uint32_t cimg_bytes = static_cast<uint32_t>(H >> 1) * (W >> 1) * a.c1 * 2;
+  auto load_patch = [&](u32x4(&pr)[PLOADS], const Step& s) __attribute__((always_inline)) {
+    const int c0 = s.ch * cc;
+    const bool coarse = c0 < a.c1;  // chunk of the nearest-upsampled coarse input
+    const uint16_t* base = coarse ? a.coarse + static_cast<size_t>(s.n) * (H >> 1) * (W >> 1) * a.c1 + c0
+                                  : a.x + static_cast<size_t>(s.n) * H * W * a.cx + (c0 - a.c1);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint16_t*>(base), static_cast<short>(0), coarse ? cimg_bytes : img_bytes, kBufFlags);
+    // byte offset of the patch corner (row y0 - 1, column x0 - 1)
+    const int corner = coarse ? (((s.y0 >> 1) - 1) * (W >> 1) + ((s.x0 >> 1) - 1)) * a.c1 * 2
+                              : ((s.y0 - 1) * W + (s.x0 - 1)) * a.cx * 2;
+    const bool interior = __builtin_amdgcn_readfirstlane(
+        s.y0 >= 1 && s.y0 + PH - 1 <= H && s.x0 >= 1 && s.x0 + PW - 1 <= W ? 1 : 0);
+    uint32_t okm = pidle;  // bit k: load k reads image data (else the zero padding)
+    if (!interior) {       // uniform branch; it only narrows the mask
+      uint32_t m = 0;
+#pragma unroll
+      for (int k = 0; k < PLOADS; ++k) {
+        const int yy = s.y0 + (pyx[k] & 0xff) - 1, xx = s.x0 + ((pyx[k] >> 8) & 0xff) - 1;
+        const bool ok = static_cast<unsigned>(yy) < static_cast<unsigned>(H) &&
+                        static_cast<unsigned>(xx) < static_cast<unsigned>(W);
+        m |= (ok ? 1u : 0u) << k;
+      }
+      okm &= m;
+    }
+#pragma unroll
+    for (int k = 0; k < PLOADS; ++k) {
+      int r0 = prel[k], r1 = crel[k];
+      asm volatile("" : "+v"(r0), "+v"(r1));  // keep the per-load select a register select
+      const int off = (okm >> k) & 1u ? corner + (coarse ? r1 : r0) : kOOB;
+      pr[k] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+    }
+  };
+  int wrel[WLOADS];
 #pragma unroll
   for (int k = 0; k < WLOADS; ++k) {
     const int i = tid + k * 256;
-    const bool ok = i < (32 * NB * 9 << gs);
-    const int qq = ok ? i >> gs : 0, gi = i & ((1 << gs) - 1), co = qq / 9, tap = qq - co * 9;
-    wsrc[k] = (co * 9 + tap) * Cin + gi * 8;
-    wlds[k] = ok ? co * kWStride + tap * 32 + gi * 8 : 32 * NB * kWStride;
+    const int qq = i >> gs, co = qq / 9, tap = qq - co * 9;
+    wrel[k] = i < (32 * NB * 9 << gs) ? ((co * 9 + tap) * Cin + (i & ((1 << gs) - 1)) * 8) * 2 : kOOB;
   }
-  u32x4 pr[PLOADS], wr[WLOADS];
-  uint32_t pvalid = 0;  // bit k: pr[k] holds image data (else zero padding)
-  float br = 0.f;
-  uint2 fr_next = make_uint2(0u, 0u), fr_cur = make_uint2(0u, 0u);  // kEpiRadiance features
-  int prev_key = -1;
-  bool wload = false;
-  auto load = [&](int it, int ch) __attribute__((always_inline)) {
-    const Tile t = decode(it);
-    const int c0 = ch * cc;
-    pvalid = 0;
-    if (c0 < a.c1) {  // chunk of the nearest-upsampled coarse input
-      const uint16_t* base = a.coarse + static_cast<size_t>(t.n) * (H >> 1) * (W >> 1) * a.c1 + c0;
+  const uint32_t w_bytes = static_cast<uint32_t>(Cout) * 9 * Cin * 2;
+  auto load_weights = [&](const Step& s) __attribute__((always_inline)) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(a.w), static_cast<short>(0), w_bytes, kBufFlags);
+    const int base = (s.co0 * 9 * Cin + s.ch * cc) * 2;
 #pragma unroll
-      for (int k = 0; k < PLOADS; ++k) {
-        const int yy = t.y0 + (pyx[k] & 0xff) - 1, xx = t.x0 + ((pyx[k] >> 8) & 0xff) - 1;
-        const bool ok = pyx[k] >= 0 && static_cast<unsigned>(yy) < static_cast<unsigned>(H) &&
-                        static_cast<unsigned>(xx) < static_cast<unsigned>(W);
-        const int off = ok ? ((yy >> 1) * (W >> 1) + (xx >> 1)) * a.c1 + ((pyx[k] >> 16) << 3) : 0;
-        pr[k] = *reinterpret_cast<const u32x4*>(base + off);
-        pvalid |= (ok ? 1u : 0u) << k;
-      }
-    } else {
-      const uint16_t* base = a.x + static_cast<size_t>(t.n) * H * W * a.cx + (c0 - a.c1);
-#pragma unroll
-      for (int k = 0; k < PLOADS; ++k) {
-        const int yy = t.y0 + (pyx[k] & 0xff) - 1, xx = t.x0 + ((pyx[k] >> 8) & 0xff) - 1;
-        const bool ok = pyx[k] >= 0 && static_cast<unsigned>(yy) < static_cast<unsigned>(H) &&
-                        static_cast<unsigned>(xx) < static_cast<unsigned>(W);
-        const int off = ok ? (yy * W + xx) * a.cx + ((pyx[k] >> 16) << 3) : 0;
-        pr[k] = *reinterpret_cast<const u32x4*>(base + off);
-        pvalid |= (ok ? 1u : 0u) << k;
-      }
-    }
-    const int key = (t.co0 / 32) * nchunks + ch;
-    wload = key != prev_key;
-    prev_key = key;
-    if (wload) {
-      const uint16_t* wb = a.w + static_cast<size_t>(t.co0) * 9 * Cin + c0;
-#pragma unroll
-      for (int k = 0; k < WLOADS; ++k) wr[k] = *reinterpret_cast<const u32x4*>(wb + wsrc[k]);
-      br = a.bias ? a.bias[t.co0 + (tid & (32 * NB - 1))] : 0.f;
-    }
-    if (EPI == kEpiRadiance) {  // single-chunk layer: features 0..3 of this lane's pixel
-      const int pi = lane & 31, yy = t.y0 + 2 * (wave * MB + (lane >> 5) % MB) + (pi >> 4);
-      const size_t p = (static_cast<size_t>(t.n) * H + (yy < H ? yy : 0)) * W + t.x0 + (pi & 15);
-      fr_next = *reinterpret_cast<const uint2*>(a.feat + p * 16);
-    }
+    for (int k = 0; k < WLOADS; ++k)
+      wr[k] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, wrel[k] == kOOB ? kOOB : base + wrel[k], 0, 0));
+    br = a.bias ? a.bias[s.co0 + (tid & (32 * NB - 1))] : 0.f;
   };
-  auto store = [&]() __attribute__((always_inline)) {
+  auto load_feat = [&](const Step& s) __attribute__((always_inline)) {  // kEpiRadiance: features 0..3
+    const int pi = lane & 31, yy = s.y0 + 2 * (wave * MB + (lane >> 5) % MB) + (pi >> 4);
+    const size_t p = (static_cast<size_t>(s.n) * H + (yy < H ? yy : 0)) * W + s.x0 + (pi & 15);
+    fr_next = *reinterpret_cast<const uint2*>(a.feat + p * 16);
+  };
+  auto store_patch = [&](const u32x4(&pr)[PLOADS]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < PLOADS; ++k)
-      *reinterpret_cast<u32x4*>(patch + plds[k]) = (pvalid >> k) & 1u ? pr[k] : u32x4{0u, 0u, 0u, 0u};
-    if (wload) {
+    for (int k = 0; k < PLOADS; ++k) *reinterpret_cast<u32x4*>(patch + plds[k]) = pr[k];
+  };
+  auto store_weights = [&]() __attribute__((always_inline)) {
 #pragma unroll
-      for (int k = 0; k < WLOADS; ++k) *reinterpret_cast<u32x4*>(wl + wlds[k]) = wr[k];
-      if (tid < 32 * NB) bl[tid] = br;
+    for (int k = 0; k < WLOADS; ++k) {
+      const int i = tid + k * 256;
+      const int qq = i >> gs, co = qq / 9, tap = qq - co * 9;
+      const int o = i < (32 * NB * 9 << gs) ? co * kWStride + tap * 32 + (i & ((1 << gs) - 1)) * 8 : 32 * NB * kWStride;
+      *reinterpret_cast<u32x4*>(wl + o) = wr[k];
     }
+    if (tid < 32 * NB) bl[tid] = br;
   };
   // Output of a finished tile, staged per wave in LDS by the epilogue ([pixel][channel]
-  // bf16) and written one step later with 16-byte stores, so that the stores are issued
-  // before the next tile's loads (one in-order counter tracks both) and never delay the
-  // wait for them. The 2x2 pooling reads the staged bf16 values back.
+  // bf16) and written one step later with 16-byte buffer stores (rows past the image
+  // fall outside the buffer and are dropped), so that the stores are issued before the
+  // following loads (one in-order counter tracks both) and never delay the wait for
+  // them. The 2x2 pooling reads the staged bf16 values back.
   uint16_t* obw = ob + wave * OWAVE;
-  auto flush = [&](int it) __attribute__((always_inline)) {
-    const Tile t = decode(it);
+  int orel[FLUSH];
+#pragma unroll
+  for (int k = 0; k < FLUSH; ++k) {
+    const int c = lane + 64 * k, px = c / PARTS, part = c - px * PARTS, pi = px & 31;
+    orel[k] = (((2 * (wave * MB + (px >> 5)) + (pi >> 4)) * W + (pi & 15)) * Cout + part * 8) * 2;
+  }
+  const uint32_t out_bytes = static_cast<uint32_t>(H) * W * Cout * 2;
+  auto flush = [&](const Step& s) __attribute__((always_inline)) {
     if (EPI == kEpiRadiance) {
       if (lane < 32 * MB) {
-        const int pi = lane & 31, yy = t.y0 + 2 * (wave * MB + (lane >> 5)) + (pi >> 4);
+        const int pi = lane & 31, yy = s.y0 + 2 * (wave * MB + (lane >> 5)) + (pi >> 4);
         const float4 y = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(obw) + lane * 4);
         if (yy < H) {
           float4 o;
@@ -211,29 +264,32 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
           o.y = expm1f(fmaxf(bf2f(static_cast<uint16_t>(fr_cur.x >> 16)) + y.y, 0.f));
           o.z = expm1f(fmaxf(bf2f(static_cast<uint16_t>(fr_cur.y & 0xffffu)) + y.z, 0.f));
           o.w = 1.f;
-          a.radiance[(static_cast<size_t>(t.n) * H + yy) * W + t.x0 + (pi & 15)] = o;
+          a.radiance[(static_cast<size_t>(s.n) * H + yy) * W + s.x0 + (pi & 15)] = o;
         }
       }
       return;
     }
-    constexpr int PARTS = 4 * NB;  // 16-byte parts per pixel
-    uint16_t* yb = static_cast<uint16_t*>(a.y) + static_cast<size_t>(t.n) * H * W * Cout + t.co0;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        static_cast<uint16_t*>(a.y) + static_cast<size_t>(s.n) * H * W * Cout + s.co0, static_cast<short>(0),
+        out_bytes, kBufFlags);
+    const int corner = (s.y0 * W + s.x0) * Cout * 2;
 #pragma unroll
-    for (int k = 0; k < (MB * 32 * PARTS) / 64; ++k) {
+    for (int k = 0; k < FLUSH; ++k) {
       const int c = lane + 64 * k, px = c / PARTS, part = c - px * PARTS;
-      const int pi = px & 31, yy = t.y0 + 2 * (wave * MB + (px >> 5)) + (pi >> 4);
       const u32x4 v = *reinterpret_cast<const u32x4*>(obw + px * OS + part * 8);
-      if (yy < H) *reinterpret_cast<u32x4*>(yb + (yy * W + t.x0 + (pi & 15)) * Cout + part * 8) = v;
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32_t, v), rs, corner + orel[k], 0, 0);
     }
     if (EPI == kEpiReluPool) {  // H even: pooled rows are whole
-      uint16_t* pb = a.pooled + static_cast<size_t>(t.n) * (H >> 1) * (W >> 1) * Cout + t.co0;
+      const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+          a.pooled + static_cast<size_t>(s.n) * (H >> 1) * (W >> 1) * Cout + s.co0, static_cast<short>(0),
+          out_bytes >> 2, kBufFlags);
       for (int c = lane; c < MB * 8 * PARTS; c += 64) {
         const int pq = c / PARTS, part = c - pq * PARTS, m = pq >> 3, px = 2 * (pq & 7);
-        const int yy = t.y0 + 2 * (wave * MB + m);
-        const uint16_t* s = obw + (m * 32 + px) * OS + part * 8;
-        const u32x4 v00 = *reinterpret_cast<const u32x4*>(s), v01 = *reinterpret_cast<const u32x4*>(s + OS);
-        const u32x4 v10 = *reinterpret_cast<const u32x4*>(s + 16 * OS);
-        const u32x4 v11 = *reinterpret_cast<const u32x4*>(s + 17 * OS);
+        const int yy = s.y0 + 2 * (wave * MB + m);
+        const uint16_t* src = obw + (m * 32 + px) * OS + part * 8;
+        const u32x4 v00 = *reinterpret_cast<const u32x4*>(src), v01 = *reinterpret_cast<const u32x4*>(src + OS);
+        const u32x4 v10 = *reinterpret_cast<const u32x4*>(src + 16 * OS);
+        const u32x4 v11 = *reinterpret_cast<const u32x4*>(src + 17 * OS);
         u32x4 o;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
@@ -242,66 +298,68 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
                            (__uint_as_float(v10[d] << 16) + __uint_as_float(v11[d] << 16));
           const float hi = (__uint_as_float(v00[d] & 0xffff0000u) + __uint_as_float(v01[d] & 0xffff0000u)) +
                            (__uint_as_float(v10[d] & 0xffff0000u) + __uint_as_float(v11[d] & 0xffff0000u));
-          o[d] = static_cast<uint32_t>(f2bf(0.25f * lo)) | (static_cast<uint32_t>(f2bf(0.25f * hi)) << 16);
+          o[d] = pack_bf16(0.25f * lo, 0.25f * hi);
         }
-        if (yy < H)
-          *reinterpret_cast<u32x4*>(pb + ((yy >> 1) * (W >> 1) + (t.x0 >> 1) + (px >> 1)) * Cout + part * 8) = o;
+        const int off = (((yy >> 1) * (W >> 1) + (s.x0 >> 1) + (px >> 1)) * Cout + part * 8) * 2;
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32_t, o), rp, off, 0, 0);
       }
     }
   };
 
   const int l32 = lane & 31, khalf = (lane >> 5) * 8;
   bool pend = false;
-  int pend_item = 0;
-  load(item, 0);
-  for (;;) {
-    const int cur = item, cur_chunk = chunk;
-    if (++chunk == nchunks) {
-      chunk = 0;
-      item += per;
-    }
-    const bool more = item < end;
+  Step pend_step;
+  // one step on the patch staged in `pr`: stage it, write the previous tile, issue the
+  // loads of the following steps, then MFMAs and the epilogue of a finished tile
+  auto step = [&](u32x4(&pr)[PLOADS], const Step& s0, const Step& s1, const Step& s2, bool valid)
+                  __attribute__((always_inline)) {
     __syncthreads();
-    store();
+    store_patch(pr);
+    if (!SC) store_weights();
     __syncthreads();
     if (pend) {
-      flush(pend_item);
+      flush(pend_step);
       pend = false;
     }
     if (EPI == kEpiRadiance) fr_cur = fr_next;
-    if (more) load(item, chunk);
+    if (!SC) load_weights(s1);
+    if (EPI == kEpiRadiance) load_feat(s1);  // single-chunk: step s1 is the next item
+    // issue order = wait order: nothing of the next step's loads may move behind these
+    __builtin_amdgcn_sched_barrier(0);
+    load_patch(pr, s2);
     // MFMA with the weights as A (rows = output channels) and the patch as B (columns =
     // pixels): accumulator register r of lane l then holds pixel l % 32 of the m-block
     // and channel 8 (r / 4) + 4 (l / 32) + r % 4 of the n-block, i.e. four runs of four
-    // consecutive channels of one pixel.
+    // consecutive channels of one pixel. Fragments are read from LDS two k-steps ahead
+    // of the MFMAs that use them (three register sets), so LDS latency overlaps MFMAs.
+    bf16x8 af[3][MB], bf[3][NB];
+    auto frag = [&](int k, int buf) __attribute__((always_inline)) {
+      const int tap = k / KS, ks = (k % KS) * 16, ky = tap / 3, kx = tap % 3;
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-      const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+      for (int m = 0; m < MB; ++m) {
+        const int row = 2 * (wave * MB + m) + (l32 >> 4) + ky, col = (l32 & 15) + kx;
+        af[buf][m] = *reinterpret_cast<const bf16x8*>(patch + row * kRowPitch + col * kPosStride + ks + khalf);
+      }
 #pragma unroll
-      for (int ks = 0; ks < 32; ks += 16) {
-        if (ks >= cc) break;
-        bf16x8 av[MB], bv[NB];
+      for (int j = 0; j < NB; ++j)
+        bf[buf][j] = *reinterpret_cast<const bf16x8*>(wl + (j * 32 + l32) * kWStride + tap * 32 + ks + khalf);
+    };
+    frag(0, 0);
+    frag(1, 1);
 #pragma unroll
-        for (int m = 0; m < MB; ++m) {
-          const int row = 2 * (wave * MB + m) + (l32 >> 4) + ky, col = (l32 & 15) + kx;
-          av[m] = *reinterpret_cast<const bf16x8*>(patch + (row * PW + col) * kPosStride + ks + khalf);
-        }
+    for (int k = 0; k < NK; ++k) {
+      if (k + 2 < NK) frag(k + 2, (k + 2) % 3);
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
 #pragma unroll
         for (int j = 0; j < NB; ++j)
-          bv[j] = *reinterpret_cast<const bf16x8*>(wl + (j * 32 + l32) * kWStride + tap * 32 + ks + khalf);
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-          for (int j = 0; j < NB; ++j)
-            acc[m][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[j], av[m], acc[m][j], 0, 0, 0);
-      }
+          acc[m][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf[k % 3][j], af[k % 3][m], acc[m][j], 0, 0, 0);
     }
-    if (cur_chunk == nchunks - 1) {
-      const Tile t = decode(cur);
+    if (valid && s0.ch == nchunks - 1) {
       const int pi = l32, h4 = 4 * (lane >> 5);
 #pragma unroll
       for (int m = 0; m < MB; ++m) {
-        const int yy = t.y0 + 2 * (wave * MB + m) + (pi >> 4);
+        const int yy = s0.y0 + 2 * (wave * MB + m) + (pi >> 4);
 #pragma unroll
         for (int j = 0; j < NB; ++j)
 #pragma unroll
@@ -319,32 +377,60 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
             if (EPI == kEpiF32) {
               if (yy < H)
                 *reinterpret_cast<float4*>(static_cast<float*>(a.y) +
-                                           ((static_cast<size_t>(t.n) * H + yy) * W + t.x0 + (pi & 15)) * Cout +
-                                           t.co0 + col) = make_float4(v0, v1, v2, v3);
+                                           ((static_cast<size_t>(s0.n) * H + yy) * W + s0.x0 + (pi & 15)) * Cout +
+                                           s0.co0 + col) = make_float4(v0, v1, v2, v3);
             } else if (EPI == kEpiRadiance) {
               if (j == 0 && qd == 0 && h4 == 0)
                 *reinterpret_cast<float4*>(reinterpret_cast<float*>(obw) + (m * 32 + pi) * 4) =
                     make_float4(v0, v1, v2, v3);
             } else {
               uint2 o;
-              o.x = static_cast<uint32_t>(f2bf(v0)) | (static_cast<uint32_t>(f2bf(v1)) << 16);
-              o.y = static_cast<uint32_t>(f2bf(v2)) | (static_cast<uint32_t>(f2bf(v3)) << 16);
+              o.x = pack_bf16(v0, v1);
+              o.y = pack_bf16(v2, v3);
               *reinterpret_cast<uint2*>(obw + (m * 32 + pi) * OS + col) = o;
             }
-            acc[m][j][4 * qd] = 0.f;
-            acc[m][j][4 * qd + 1] = 0.f;
-            acc[m][j][4 * qd + 2] = 0.f;
-            acc[m][j][4 * qd + 3] = 0.f;
           }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[m][j] = f32x16{};
       }
       if (EPI != kEpiF32) {
         pend = true;
-        pend_item = cur;
+        pend_step = s0;
       }
     }
-    if (!more) break;
+  };
+
+  // step cursors: s0 is computed now, s1 and s2 follow. The loop runs pairs of steps
+  // (register sets A, B) with its only exit test at the end; an odd last step is a B
+  // step that computes a repeat of the last staged data and discards it.
+  const int n_steps = ((end - first + per - 1) / per) * nchunks;
+  Step s0;
+  s0.ch = 0;
+  set_item(s0, first);
+  Step s1 = advance(s0), s2 = advance(s1);
+  if (SC) {  // weights and bias once
+    load_weights(s0);
+    __syncthreads();
+    store_weights();
+  } else {
+    load_weights(s0);
   }
-  if (pend) flush(pend_item);
+  if (EPI == kEpiRadiance) load_feat(s0);
+  __builtin_amdgcn_sched_barrier(0);
+  load_patch(prA, s0);
+  __builtin_amdgcn_sched_barrier(0);
+  load_patch(prB, s1);
+  for (int k = 0; k < n_steps; k += 2) {
+    step(prA, s0, s1, s2, true);
+    s0 = s1;
+    s1 = s2;
+    s2 = advance(s2);
+    step(prB, s0, s1, s2, k + 1 < n_steps);
+    s0 = s1;
+    s1 = s2;
+    s2 = advance(s2);
+  }
+  if (pend) flush(pend_step);
 }
 
 // ------------------------------------------------------------------------------------
@@ -654,8 +740,17 @@ static unsigned persistent_blocks(K kernel, long items, int groups) {
 
 template <int MB, int NB, int EPI>
 static void launch_conv_epi(const ConvArgs& a, long items, hipStream_t s) {
-  const unsigned blocks = persistent_blocks(k_conv<MB, NB, EPI>, items, a.groups);
-  hipLaunchKernelGGL((k_conv<MB, NB, EPI>), dim3(blocks), dim3(256), 0, s, a);
+  const int cin = a.c1 + a.cx;
+  if (cin == 16) {
+    const unsigned blocks = persistent_blocks(k_conv<MB, NB, EPI, true, true>, items, a.groups);
+    hipLaunchKernelGGL((k_conv<MB, NB, EPI, true, true>), dim3(blocks), dim3(256), 0, s, a);
+  } else if (cin == 32) {
+    const unsigned blocks = persistent_blocks(k_conv<MB, NB, EPI, true, false>, items, a.groups);
+    hipLaunchKernelGGL((k_conv<MB, NB, EPI, true, false>), dim3(blocks), dim3(256), 0, s, a);
+  } else {
+    const unsigned blocks = persistent_blocks(k_conv<MB, NB, EPI, false, false>, items, a.groups);
+    hipLaunchKernelGGL((k_conv<MB, NB, EPI, false, false>), dim3(blocks), dim3(256), 0, s, a);
+  }
 }
 
 template <int MB, int NB>
