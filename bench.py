@@ -36,7 +36,29 @@ def parse():
                     help="BVH spatial pre-splitting budget, extra references per triangle (0 = off)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--denoise-only", action="store_true",
+                    help="time only U-Net inference on a 1920x1080 1-spp Cornell frame (untrained weights; "
+                         "--steps timed calls after --warmup)")
     return ap.parse_args()
+
+
+def denoise_only(steps, warmup):
+    """Inference time of the denoiser on one 1080p frame; the network's cost does not
+    depend on its weights, so freshly initialised ones are used."""
+    from trayracer_amd import Renderer, scenes
+    rend = Renderer(0)
+    rend.UpdateScene(scenes.cornell(width=1920, height=1080, spp=1))
+    rend.RenderFrame()
+    rend.denoiser_init(seed=0)
+    for _ in range(warmup):
+        rend.denoise()
+    st = [rend.denoise() for _ in range(steps)]
+    n = len(st)
+    print(json.dumps({"workload": "U-Net denoise, 1920x1080", "steps": n, "warmup": warmup,
+                      "denoise_ms": round(sum(s["total_ms"] for s in st) / n, 4),
+                      "features_ms": round(sum(s["features_ms"] for s in st) / n, 4),
+                      "network_ms": round(sum(s["network_ms"] for s in st) / n, 4)}), flush=True)
+    rend.close()
 
 
 def copy_calibrate():
@@ -118,6 +140,9 @@ def main():
     args = parse()
     if args.copy_calibrate:
         copy_calibrate()
+        return
+    if args.denoise_only:
+        denoise_only(args.steps, args.warmup)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: every configuration runs on one MI355X; "
