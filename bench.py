@@ -103,6 +103,31 @@ def cpu_baseline(scene, threads, spp=2):
     }
 
 
+# U-Net convolutions (input channels, output channels, level): encoder 2 per level, decoder
+# [upsample | skip] + 1, then the 3-channel output layer (stored padded to 32 channels)
+UNET_CONVS = [(16, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 96, 2), (96, 96, 2), (96, 128, 3),
+              (128, 128, 3), (224, 96, 2), (96, 96, 2), (160, 64, 1), (64, 64, 1), (96, 32, 0), (32, 32, 0),
+              (32, 3, 0)]
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def unet_flops(width, height):
+    """Algorithmic FLOPs of one inference: 2 x 9 x Cin x Cout per output pixel per conv
+    (the 16 input channels as stored, 10 of them used; the output layer's 3 channels)."""
+    return sum(2.0 * 9 * ci * co * (width >> lvl) * (height >> lvl) for ci, co, lvl in UNET_CONVS)
+
+
+def network_roofline(rend, calls=10):
+    """MFMA roofline of the network on the frame currently held (1920x1080)."""
+    ms = [rend.denoise()["network_ms"] for _ in range(calls)]
+    t = sum(ms) / len(ms)
+    flops = unet_flops(1920, 1080)
+    achieved = flops / (t * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "flops_per_inference": flops,
+            "network_ms": round(t, 4), "profile": "profiles/unet_denoise_1080p.md"}
+
+
 def denoise_metric(rend, steps):
     """Time to relMSE 0.01 against a 4096-spp render, plain accumulation vs learned
     denoising, on the Cornell box at 1920x1080 seen from its own (held-out) camera. The
@@ -133,7 +158,7 @@ def denoise_metric(rend, steps):
             "relmse": "mean (x - r)^2 / (r^2 + 1e-2) vs a 4096-spp render", **pick(res),
             "train": {"pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1), "steps": steps, "batch": 8,
                       "crop": 128, "final_loss": float(losses[-50:].mean())},
-            "instanced_view": pick(inst)}
+            "instanced_view": pick(inst), "network_roofline": network_roofline(rend)}
 
 
 def main():
