@@ -56,8 +56,9 @@ __device__ __forceinline__ void wave_add_stat(unsigned long long* dst, uint32_t 
 // wave: one atomic on the fetch cursor of the first segment, from `home` on, not yet
 // found exhausted. `n_seg` holds, in lane s < kSegs, the item count of segment s
 // (constant during the kernel); `done` collects exhausted segments (cursors only grow,
-// so they stay exhausted; callers start it with seg_empty()). Every wave makes at most
-// kSegs failed attempts, so the loop always ends. The cursors are not pre-read: coherent
+// so they stay exhausted; callers start it with seg_empty(), which also closes the
+// segments outside the wave's fetch window). Every wave makes at most kFetchWindow
+// failed attempts (kSegs in grids of fewer than kSegs waves), so the loop always ends. The cursors are not pre-read: coherent
 // reads of a hot line serialise like the atomics themselves (measured: a pre-read of all
 // 32 cursors per batch cost as much as the whole Shade kernel's work).
 __device__ __forceinline__ bool seg_fetch_k(const SegIn& in, uint32_t n_seg, uint32_t home, uint64_t& done,
@@ -88,9 +89,26 @@ __device__ __forceinline__ bool seg_fetch(const SegIn& in, uint32_t n_seg, uint3
   return seg_fetch_k(in, n_seg, home, done, 64u, slot0, n_valid);
 }
 
-// Segments without items: the initial `done` mask of seg_fetch (uniform per wave).
+// Home segment of the calling wave.
+__device__ __forceinline__ uint32_t wave_home() { return (blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kSegs; }
+
+// Initial `done` mask of seg_fetch (uniform per wave): segments without items, and every
+// segment outside the wave's window {home, home + 1}. A wave thus only ever touches two
+// fetch cursors; once the queue runs dry each cursor line takes (waves / kSegs) x 2
+// failed attempts instead of one from every wave (the tail of small launches was spent
+// in those serialised atomics). Every segment lies in the window of the waves whose
+// home it is, so with at least kSegs waves in the grid every item is fetched; smaller
+// grids keep all segments open.
+constexpr uint32_t kFetchWindow = 2;
 __device__ __forceinline__ uint64_t seg_empty(uint32_t n_seg) {
-  return ~__ballot(n_seg > 0u) & ((1ull << kSegs) - 1ull);
+  const uint64_t all = (1ull << kSegs) - 1ull;
+  uint64_t closed = ~__ballot(n_seg > 0u);
+  if (gridDim.x * kWavesPerBlock >= kSegs) {
+    const uint32_t home = wave_home();
+    const uint64_t w = (1ull << kFetchWindow) - 1ull;
+    closed |= ~(((w << home) | (w >> (kSegs - home))) & all);
+  }
+  return closed & all;
 }
 
 // Per-lane item count of segment `lane` (lanes >= kSegs: 0).
@@ -101,8 +119,6 @@ __device__ __forceinline__ uint32_t seg_counts(const SegIn& in) {
   return c < in.cap ? c : in.cap;
 }
 
-// Home segment of the calling wave.
-__device__ __forceinline__ uint32_t wave_home() { return (blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % kSegs; }
 
 // Appends the lanes in `mask` to a segmented queue; returns this lane's slot (kNoHit
 // for lanes not in the mask). Starts at segment `home` and moves on while segments are
