@@ -32,7 +32,7 @@ def parse():
                     help="skip the time-to-relMSE measurement (denoiser training + convergence runs)")
     ap.add_argument("--denoise-steps", type=int, default=4000)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--presplit", type=float, default=3.0,
+    ap.add_argument("--presplit", type=float, default=6.0,
                     help="BVH spatial pre-splitting budget, extra references per triangle (0 = off)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")

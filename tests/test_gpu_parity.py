@@ -54,7 +54,7 @@ def test_trace_cornell_exact(rend, presplit):
     assert np.array_equal(occ_g, occ_r)
 
 
-@pytest.mark.parametrize("presplit", [0.0, 1.0, 3.0])
+@pytest.mark.parametrize("presplit", [0.0, 1.0, 3.0, 6.0])
 def test_trace_soup_exact(rend, presplit):
     sc = scenes.soup(n_tris=200_000, seed=5, width=64, height=64)
     rend.UpdateScene(sc, presplit=presplit)
@@ -105,7 +105,7 @@ def test_path_radiance_cornell_rr_deep(rend):
     _compare_paths(rend.path_radiance(), ref["path_rgb"], "cornell rr")
 
 
-@pytest.mark.parametrize("presplit", [0.0, 1.0])
+@pytest.mark.parametrize("presplit", [0.0, 1.0, 6.0])
 def test_path_radiance_soup(rend, presplit):
     sc = scenes.soup(n_tris=50_000, seed=3, width=48, height=32, spp=2, max_depth=8)
     rend.UpdateScene(sc, presplit=presplit)

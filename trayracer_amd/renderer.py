@@ -88,7 +88,7 @@ class Renderer:
             raise TrtError(st, self.lib.trt_last_error(self.ctx).decode())
 
     # -- scene -----------------------------------------------------------------------
-    def UpdateScene(self, scene, bins=32, max_leaf=4, threads=0, presplit=3.0):
+    def UpdateScene(self, scene, bins=32, max_leaf=4, threads=0, presplit=6.0):
         """Uploads a scene dict (see trayracer_amd.scenes) and builds its BVH(s).
 
         presplit: spatial pre-splitting budget in extra references per triangle (0 = off).
@@ -123,7 +123,7 @@ class Renderer:
         if "render" in scene:
             self.SetSettings(KernelSettings.from_dict(scene["render"]))
 
-    def LoadScene(self, path, bins=32, max_leaf=4, threads=0, presplit=3.0):
+    def LoadScene(self, path, bins=32, max_leaf=4, threads=0, presplit=6.0):
         """Loads a scene file in the reference's JSON format (OBJ models) and builds it
         (SceneFile::Load, Tracer/FileIO/SceneFile.cpp:683-731). The file's render settings
         (multisample, maxdepth, ray-epsilon) replace the current settings; the film size,
