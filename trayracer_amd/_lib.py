@@ -94,7 +94,7 @@ class TrtStats(ctypes.Structure):
         ("raygen_ms", ctypes.c_float), ("extend_ms", ctypes.c_float), ("shade_ms", ctypes.c_float),
         ("shadow_ms", ctypes.c_float), ("finalize_ms", ctypes.c_float), ("frame_ms", ctypes.c_float),
         ("extend_launches", ctypes.c_uint32), ("shadow_launches", ctypes.c_uint32),
-        ("sample_count", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
+        ("sample_count", ctypes.c_uint32), ("overflow", ctypes.c_uint32), ("trace_ms", ctypes.c_float),
     ]
 
     def as_dict(self):

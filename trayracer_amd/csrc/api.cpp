@@ -115,7 +115,14 @@ struct trt_context {
   uint32_t seg_cap = 0;      // slots per queue segment for the current frame size
 
   hipEvent_t ev[4 + 4 * kMaxDepth] = {};
+  // Shadow rays of bounce d run on a second stream, concurrently with Extend of bounce
+  // d + 1 (they share no buffer: Shadow adds to path radiance, Extend writes hits);
+  // Shade of bounce d + 1 waits for them, which keeps every path's radiance additions
+  // in the serial order. ev2: per bounce, Shadow start and end on that stream.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev2[2 * kMaxDepth] = {};
   int grid_trace = 0, grid_shade = 0;
+  size_t spill_half = 0;  // d_spill holds two areas: Extend's and the concurrent Shadow's
 
   // differentiable rendering
   std::vector<Material> sampling_materials;  // empty = same as materials
@@ -363,6 +370,12 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
                                                         o.mode == kModeAmbientOcclusionShading) ? 2u : 1u);
 
   const uint32_t cap = ctx->seg_cap;
+  // the concurrent Shadow launches use the second spill area
+  DevAccel acc_sh = acc;
+  acc_sh.spill = acc.spill + ctx->spill_half;
+  hipStream_t stm2 = ctx->stream2;
+  hipEvent_t* ev2 = ctx->ev2;
+
   launch_zero_counters(ctr, stm);
   HIPC(hipEventRecord(ev[0], stm), "event");
   launch_raygen(rp, n, cap, q[0], fb.radiance, ctr, stm);
@@ -373,14 +386,23 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
     launch_extend(acc, q[cur], hb, SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkExt, di), cap}, ctr,
                   ctx->grid_trace, stats, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 0], stm), "event");
+    // Shade adds to the radiance that the previous bounce's Shadow adds to, and reuses
+    // its shadow queue: wait for it
+    if (d > 0) HIPC(hipStreamWaitEvent(stm, ev2[2 * (d - 1) + 1], 0), "wait");
+    HIPC(hipEventRecord(ev[4 + 4 * d + 3], stm), "event");
     launch_shade(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq, fb,
                  SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkShade, di), cap},
                  ctr + seg_ctr(kSegExt, di + 1), ctr + seg_ctr(kSegShadow, di), ctr, d, ctx->grid_shade, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 1], stm), "event");
-    launch_shadow(acc, sq, fb.radiance, SegIn{ctr + seg_ctr(kSegShadow, di), ctr + seg_ctr(kSegWorkShadow, di), cap},
-                  ctr, ctx->grid_trace, stats, stm);
-    HIPC(hipEventRecord(ev[4 + 4 * d + 2], stm), "event");
+    HIPC(hipStreamWaitEvent(stm2, ev[4 + 4 * d + 1], 0), "wait");
+    HIPC(hipEventRecord(ev2[2 * d], stm2), "event");
+    launch_shadow(acc_sh, sq, fb.radiance,
+                  SegIn{ctr + seg_ctr(kSegShadow, di), ctr + seg_ctr(kSegWorkShadow, di), cap}, ctr, ctx->grid_trace,
+                  stats, stm2);
+    HIPC(hipEventRecord(ev2[2 * d + 1], stm2), "event");
   }
+  if (D > 0) HIPC(hipStreamWaitEvent(stm, ev2[2 * (D - 1) + 1], 0), "wait");
+  HIPC(hipEventRecord(ev[3], stm), "event");
   if (accumulate) launch_finalize(rp, fb.radiance, fb, ctx->sample_count == 0 ? 1u : 0u, stm);
   HIPC(hipEventRecord(ev[2], stm), "event");
   launch_sum_counts(ctr, cap, stm);
@@ -405,20 +427,23 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
     st->raygen_ms = ms;
+    // per-kernel intervals; Extend and Shadow overlap, so the phases add up to more than
+    // the frame
     hipEvent_t prev = ev[1];
     for (uint32_t d = 0; d < D; ++d) {
       (void)hipEventElapsedTime(&ms, prev, ev[4 + 4 * d + 0]);
       st->extend_ms += ms;
-      (void)hipEventElapsedTime(&ms, ev[4 + 4 * d + 0], ev[4 + 4 * d + 1]);
+      (void)hipEventElapsedTime(&ms, ev[4 + 4 * d + 3], ev[4 + 4 * d + 1]);
       st->shade_ms += ms;
-      (void)hipEventElapsedTime(&ms, ev[4 + 4 * d + 1], ev[4 + 4 * d + 2]);
+      (void)hipEventElapsedTime(&ms, ev2[2 * d], ev2[2 * d + 1]);
       st->shadow_ms += ms;
-      prev = ev[4 + 4 * d + 2];
+      prev = ev[4 + 4 * d + 1];
     }
-    (void)hipEventElapsedTime(&ms, prev, ev[2]);
+    (void)hipEventElapsedTime(&ms, D > 0 ? ev[3] : ev[1], ev[2]);
     st->finalize_ms = ms;
     (void)hipEventElapsedTime(&ms, ev[0], ev[2]);
     st->frame_ms = ms;
+    st->trace_ms = st->frame_ms - st->raygen_ms - st->shade_ms - st->finalize_ms;
     st->extend_launches = D;
     st->shadow_launches = D;
     st->sample_count = ctx->sample_count;
@@ -441,10 +466,12 @@ const char* trt_version(void) { return "trt 0.1 (gfx950 wavefront path tracer)";
 static trt_status init_context(trt_context* ctx) {
   HIPC(hipSetDevice(ctx->device), "hipSetDevice");
   HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
+  HIPC(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking), "stream");
   hipDeviceProp_t prop;
   HIPC(hipGetDeviceProperties(&prop, ctx->device), "device properties");
   ctx->num_cus = prop.multiProcessorCount;
   for (auto& e : ctx->ev) HIPC(hipEventCreate(&e), "event");
+  for (auto& e : ctx->ev2) HIPC(hipEventCreate(&e), "event");
   HIPC(ctx->counters.ensure(kNumCounters * 4), "alloc counters");
   HIPC(hipMemset(ctx->counters.p, 0, kNumCounters * 4), "clear counters");
   return TRT_OK;
@@ -471,9 +498,13 @@ void trt_destroy(trt_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->ev2)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   delete ctx;
 }
 
@@ -708,8 +739,10 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   // persistent grid sizes from occupancy (every resident slot gets a wave)
   ctx->grid_trace = ctx->num_cus * trace_blocks_per_cu(accel_of(ctx));
   ctx->grid_shade = ctx->num_cus * 8;
-  // stack spill areas of the single-level traversal: stack_cap entries per grid thread
-  HIPC(ctx->d_spill.ensure(size_t(ctx->grid_trace) * 256 * size_t(ctx->stack_cap) * 4), "alloc stack spill");
+  // stack spill areas of the traversals: stack_cap entries per grid thread, twice (Extend
+  // and the Shadow launch that runs beside it)
+  ctx->spill_half = size_t(ctx->grid_trace) * 256 * size_t(ctx->stack_cap);
+  HIPC(ctx->d_spill.ensure(2 * ctx->spill_half * 4), "alloc stack spill");
 
   const auto t1 = std::chrono::steady_clock::now();
   ctx->info = trt_bvh_info{};
