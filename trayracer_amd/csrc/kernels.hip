@@ -385,10 +385,11 @@ __device__ __forceinline__ bool traverse_ww(const DevNode2* __restrict__ nodes, 
 // ("dynamic fetch", Aila & Laine 2009): the wave runs rounds of the speculative
 // while-while loop above, and before each round lanes whose ray has finished take new
 // rays from the queue once at least kRefill of them are idle, so a few long rays no
-// longer hold the whole wave (16 to 32 of 64 measured best on MI355X; 64, i.e. no
-// refill, is 18% slower). load(slot, o, d, tmin, tmax) reads a ray;
+// longer hold the whole wave (32 to 40 of 64 measured best on MI355X with the early
+// end of the node loop, 24 and 48 about 1% slower; 64, i.e. no refill, was 18% slower).
+// load(slot, o, d, tmin, tmax) reads a ray;
 // finish(slot, r, hit) consumes a result (hit: a hit was found).
-constexpr uint32_t kRefill = 24;
+constexpr uint32_t kRefill = 32;
 
 // Traversal stack of the single-level kernels: the top kRingEntries entries live in an
 // LDS ring (entry k in slot k % kRingEntries, lane-interleaved), older ones are moved
@@ -398,15 +399,28 @@ constexpr uint32_t kRefill = 24;
 constexpr int kRingEntries = 16;
 
 struct TwoTierStack {
-  uint32_t* ring;  // lane's first ring slot (slots 64 words apart)
-  uint32_t* spill; // lane's spill area (cap entries)
-  int sp;          // entries on the stack
-  int lo;          // entries [0, lo) are in the spill area, [lo, sp) in the ring
+  uint32_t* ring;        // lane's first ring slot (slots 64 words apart)
+  uint32_t* spill_wave;  // the wave's spill area; lane l's `cap` entries start at l * cap
+  int cap;
+  int sp;                // entries on the stack
+  int lo;                // entries [0, lo) are in the spill area, [lo, sp) in the ring
 };
+
+// The wave's spill area of the traversal grid (stack_cap entries per thread), taken
+// wave-uniform so that it stays in scalar registers; a lane's own area is formed only
+// on the rare ring overflow or underflow.
+__device__ __forceinline__ uint32_t* wave_spill(uint32_t* spill, int cap) {
+  const uint32_t wave_first = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+  return spill + static_cast<size_t>(blockIdx.x * kBlock + wave_first) * static_cast<uint32_t>(cap);
+}
+
+__device__ __forceinline__ uint32_t* lane_spill(const TwoTierStack& s) {
+  return s.spill_wave + lane_id() * static_cast<uint32_t>(s.cap);
+}
 
 __device__ __forceinline__ void stack_push(TwoTierStack& s, uint32_t v) {
   if (s.sp - s.lo == kRingEntries) {  // ring full: move its oldest entry out
-    s.spill[s.lo] = s.ring[(s.lo & (kRingEntries - 1)) * 64];
+    lane_spill(s)[s.lo] = s.ring[(s.lo & (kRingEntries - 1)) * 64];
     ++s.lo;
   }
   s.ring[(s.sp & (kRingEntries - 1)) * 64] = v;
@@ -417,16 +431,59 @@ __device__ __forceinline__ int32_t stack_pop(TwoTierStack& s) {
   if (s.sp == 0) return kTravDone;
   if (s.sp == s.lo) {  // ring empty: bring back up to a ring of spilled entries
     const int n = s.lo < kRingEntries ? s.lo : kRingEntries;
-    for (int k = s.lo - n; k < s.lo; ++k) s.ring[(k & (kRingEntries - 1)) * 64] = s.spill[k];
+    const uint32_t* sp = lane_spill(s);
+    for (int k = s.lo - n; k < s.lo; ++k) s.ring[(k & (kRingEntries - 1)) * 64] = sp[k];
     s.lo -= n;
   }
   --s.sp;
   return static_cast<int32_t>(s.ring[(s.sp & (kRingEntries - 1)) * 64]);
 }
 
+// Leaf phase of the single-level loop, spread over the whole wave: every (ray, triangle)
+// pair of the leaves the lanes hold is a task, and the wave deals the tasks out 64 at a
+// time, each lane testing one with the owning lane's ray (read across lanes), instead of
+// each lane looping over its own leaves while most lanes wait (a round's leaves hold
+// ~120 triangles, which the per-lane loop tested at ~16 active lanes per iteration).
+// An owner's closest hit is the minimum of a (t, prim) key over its tasks (LDS atomic
+// min; the lexicographic order of the closest-hit search), and the task that holds the
+// minimum writes the barycentrics. Per wave, in LDS: one window of kTaskWin task
+// triangles and owners per round, and a key and barycentrics per lane.
+constexpr uint32_t kTaskWin = 128;
+constexpr uint32_t kTaskWords = kTaskWin + kTaskWin / 4 + 64 * 2 + 64 * 2;
+
+struct LeafTasks {
+  uint32_t* tri;            // [kTaskWin] triangle index
+  uint8_t* own;             // [kTaskWin] owning lane
+  unsigned long long* key;  // [64] hit key per lane
+  float2* bary;             // [64] barycentrics of the key's hit
+};
+
+__device__ __forceinline__ LeafTasks leaf_tasks(uint32_t* base) {
+  LeafTasks t;
+  t.tri = base;
+  t.own = reinterpret_cast<uint8_t*>(base + kTaskWin);
+  t.key = reinterpret_cast<unsigned long long*>(base + kTaskWin + kTaskWin / 4);
+  t.bary = reinterpret_cast<float2*>(base + kTaskWin + kTaskWin / 4 + 128);
+  return t;
+}
+
+// Order-preserving (t, prim) key: t's bits mapped so that unsigned order is float
+// order (-0 taken as +0), prim below.
+__device__ __forceinline__ unsigned long long hit_key(float t, uint32_t prim) {
+  uint32_t u = __float_as_uint(t + 0.f);
+  u ^= (u >> 31) ? 0xffffffffu : 0x80000000u;
+  return (static_cast<unsigned long long>(u) << 32) | prim;
+}
+
+__device__ __forceinline__ float key_t(unsigned long long k) {
+  uint32_t u = static_cast<uint32_t>(k >> 32);
+  u ^= (u >> 31) ? 0x80000000u : 0xffffffffu;
+  return __uint_as_float(u);
+}
+
 template <bool ANY, bool STATS, class Load, class Finish>
-__device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in, uint32_t* stk, uint32_t* overflow,
-                                            TraceOut& stats, Load load, Finish finish) {
+__device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in, uint32_t* stk, uint32_t* tsk,
+                                            uint32_t* overflow, TraceOut& stats, Load load, Finish finish) {
   const uint32_t lane = lane_id();
   const uint32_t n_seg = seg_counts(in), home = wave_home();
   const int cap = acc.stack_cap;
@@ -438,7 +495,8 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
   V3 o = v3(0.f), d = v3(0.f);
   RayBox rb{};
   float tmin = 0.f;
-  RayTri rt{};
+  int32_t kxyz = 0;        // ray_tri_setup's axis permutation, 2 bits per axis
+  V3 shear = v3(0.f);      // and its shear constants
   TraceOut r;
   r.t = 0.f;
   r.b1 = r.b2 = 0.f;
@@ -446,11 +504,13 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
   r.inst = kNoInstance;
   TwoTierStack st;
   st.ring = stk;
-  st.spill = acc.spill + static_cast<size_t>(blockIdx.x * kBlock + threadIdx.x) * static_cast<uint32_t>(cap);
+  st.spill_wave = wave_spill(acc.spill, cap);
+  st.cap = cap;
   st.sp = 0;
   st.lo = 0;
   int32_t node = kTravDone, leaf = 0;
   auto pop = [&]() -> int32_t { return stack_pop(st); };
+  const LeafTasks tk = leaf_tasks(tsk);
   for (;;) {
     const uint64_t idle = __ballot(slot == kNoHit);
     const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
@@ -462,10 +522,11 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
           slot = base + rank;
           float tmax;
           load(slot, o, d, tmin, tmax);
-          rt = ray_tri_setup(o, d);
+          const RayTri rt = ray_tri_setup(o, d);
+          kxyz = rt.kx | (rt.ky << 2) | (rt.kz << 4);
+          shear = v3(rt.sx, rt.sy, rt.sz);
           rb = ray_box_setup(o, d);
           r.t = tmax;
-          r.b1 = r.b2 = 0.f;
           r.prim = kNoHit;
           r.inst = kNoInstance;
           st.sp = 0;
@@ -478,74 +539,114 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
       }
     }
     if (!__any(slot != kNoHit)) break;  // drained and every lane idle
-    if (slot == kNoHit) continue;
-    bool hit = false;
-    // inner nodes
-    while (node >= 0) {
-      float4 r0, r1, r2;
-      int4 c;
-      load_node(nodes, node, r0, r1, r2, c);
-      if (STATS) stats.nodes++;
-      float t0, t1;
-      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
-      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
-      if (h0 && h1) {
-        int32_t nearc = c.x, farc = c.y;
-        if (t1 < t0) {
-          nearc = c.y;
-          farc = c.x;
-        }
-        if (st.sp < cap) {
-          stack_push(st, static_cast<uint32_t>(farc));
+    if (slot != kNoHit) {
+      // inner nodes
+      while (node >= 0) {
+        float4 r0, r1, r2;
+        int4 c;
+        load_node(nodes, node, r0, r1, r2, c);
+        if (STATS) stats.nodes++;
+        float t0, t1;
+        const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+        const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+        if (h0 && h1) {
+          int32_t nearc = c.x, farc = c.y;
+          if (t1 < t0) {
+            nearc = c.y;
+            farc = c.x;
+          }
+          if (st.sp < cap) {
+            stack_push(st, static_cast<uint32_t>(farc));
+          } else {
+            atomicOr(overflow, 1u);
+          }
+          node = nearc;
+        } else if (h0) {
+          node = c.x;
+        } else if (h1) {
+          node = c.y;
         } else {
-          atomicOr(overflow, 1u);
+          node = pop();
         }
-        node = nearc;
-      } else if (h0) {
-        node = c.x;
-      } else if (h1) {
-        node = c.y;
-      } else {
-        node = pop();
+        if (node < 0 && node != kTravDone && leaf == 0) {
+          leaf = node;
+          node = pop();
+        }
+        if (searching_lanes(leaf) <= kSearchBreak) break;
       }
-      if (node < 0 && node != kTravDone && leaf == 0) {
-        leaf = node;
-        node = pop();
-      }
-      if (searching_lanes(leaf) <= kSearchBreak) break;
     }
-    // leaves
-    while (leaf != 0) {
-      const uint32_t code = ~static_cast<uint32_t>(leaf);
-      const uint32_t first = code >> 3, cnt = code & 7u;
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const float4* tp = tris + static_cast<size_t>(first + k) * 3u;
-        const float4 a = tp[0], b = tp[1], cc = tp[2];
-        if (STATS) stats.tests++;
-        TriHit th;
-        if (intersect_triangle(rt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), tmin, r.t, th)) {
-          const uint32_t prim = __float_as_uint(a.w);
-          if (th.t < r.t || prim < r.prim) {
-            r.t = th.t;
-            r.b1 = th.b1;
-            r.b2 = th.b2;
-            r.prim = prim;
-          }
-          if (ANY) {
-            hit = true;
-            break;
+    // leaves (the parked one, and `node` when it is a leaf), spread over the wave
+    bool hit = false;
+    const bool node_leaf = node < 0 && node != kTravDone;
+    const uint32_t ca = leaf != 0 ? (~static_cast<uint32_t>(leaf) & 7u) : 0u;
+    const uint32_t cb = node_leaf ? (~static_cast<uint32_t>(node) & 7u) : 0u;
+    const uint32_t fa = ~static_cast<uint32_t>(leaf) >> 3, fb = ~static_cast<uint32_t>(node) >> 3;
+    const uint32_t n_tasks = ca + cb;  // at most 14
+    uint32_t off = 0, total = 0;       // exclusive prefix and sum of n_tasks over the wave
+    for (uint32_t bit = 0; bit < 4; ++bit) {
+      const uint64_t m = __ballot((n_tasks >> bit) & 1u);
+      off += prefix_count(m) << bit;
+      total += static_cast<uint32_t>(__popcll(m)) << bit;
+    }
+    if (total != 0) {
+      // one window of kTaskWin tasks per round: lanes whose tasks do not fit keep their
+      // leaves for the next round (the first lane with tasks always fits)
+      const bool fits = off + n_tasks <= kTaskWin;
+      const uint64_t late = __ballot(!fits);
+      const uint32_t m = late != 0 ? __shfl(off, static_cast<int>(__builtin_ctzll(late))) : total;
+      const unsigned long long key0 = hit_key(r.t, r.prim);
+      tk.key[lane] = key0;
+      if (fits) {
+        for (uint32_t k = 0; k < n_tasks; ++k) {
+          tk.tri[off + k] = k < ca ? fa + k : fb + (k - ca);
+          tk.own[off + k] = static_cast<uint8_t>(lane);
+        }
+      }
+      for (uint32_t s0 = 0; s0 < m; s0 += 64) {
+        const uint32_t j = s0 + lane;
+        const bool act = j < m;
+        const uint32_t ti = act ? tk.tri[j] : 0u;
+        const int own = act ? static_cast<int>(tk.own[j]) : static_cast<int>(lane);
+        RayTri ort;
+        ort.o = v3(__shfl(rb.o.x, own), __shfl(rb.o.y, own), __shfl(rb.o.z, own));
+        const int kk = __shfl(kxyz, own);
+        ort.kx = kk & 3;
+        ort.ky = (kk >> 2) & 3;
+        ort.kz = kk >> 4;
+        ort.sx = __shfl(shear.x, own);
+        ort.sy = __shfl(shear.y, own);
+        ort.sz = __shfl(shear.z, own);
+        const float otmin = __shfl(tmin, own), otmax = __shfl(r.t, own);
+        if (act) {
+          const float4* tp = tris + static_cast<size_t>(ti) * 3u;
+          const float4 a = tp[0], b = tp[1], cc = tp[2];
+          if (STATS) stats.tests++;
+          TriHit th;
+          if (intersect_triangle(ort, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), otmin, otmax, th)) {
+            const unsigned long long key = hit_key(th.t, __float_as_uint(a.w));
+            atomicMin(&tk.key[own], key);
+            if (tk.key[own] == key) tk.bary[own] = make_float2(th.b1, th.b2);
           }
         }
       }
-      if (ANY && hit) break;
-      if (node < 0 && node != kTravDone) {
-        leaf = node;
-        node = pop();
-      } else {
+      if (fits && n_tasks != 0) {
+        const unsigned long long kf = tk.key[lane];
+        if (kf < key0) {  // the barycentrics stay in tk.bary[lane] until the ray finishes
+          r.t = key_t(kf);
+          r.prim = static_cast<uint32_t>(kf);
+          hit = ANY;
+        }
         leaf = 0;
+        if (node_leaf) node = pop();
       }
     }
-    if (hit || (node == kTravDone && leaf == 0)) {
+    if (slot != kNoHit && (hit || (node == kTravDone && leaf == 0))) {
+      r.b1 = r.b2 = 0.f;
+      if (r.prim != kNoHit) {
+        const float2 bb = tk.bary[lane];
+        r.b1 = bb.x;
+        r.b2 = bb.y;
+      }
       finish(slot, r, hit || r.prim != kNoHit);
       slot = kNoHit;
       node = kTravDone;
@@ -589,7 +690,8 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
   bool in_blas = false;
   TwoTierStack st;
   st.ring = stk;
-  st.spill = acc.spill + static_cast<size_t>(blockIdx.x * kBlock + threadIdx.x) * static_cast<uint32_t>(cap);
+  st.spill_wave = wave_spill(acc.spill, cap);
+  st.cap = cap;
   st.sp = 0;
   st.lo = 0;
   int32_t node = kTravDone, leaf = 0;
@@ -864,11 +966,12 @@ template <bool STATS>
 __global__ __launch_bounds__(kBlock) void k_extend_sl(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
   uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
+  uint32_t* tsk = lds + kWavesPerBlock * kRingEntries * 64 + (threadIdx.x >> 6) * kTaskWords;
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
   trace_queue<false, STATS>(
-      acc, in, stk, &ctr[kCtrOverflow], st,
+      acc, in, stk, tsk, &ctr[kCtrOverflow], st,
       [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
         const float4 a = q.o_tmin[slot];
         const float4 b = q.d_tmax[slot];
@@ -891,11 +994,12 @@ __global__ __launch_bounds__(kBlock) void k_shadow_sl(DevAccel acc, ShadowQueue 
                                                        uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
   uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
+  uint32_t* tsk = lds + kWavesPerBlock * kRingEntries * 64 + (threadIdx.x >> 6) * kTaskWords;
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
   trace_queue<true, STATS>(
-      acc, in, stk, &ctr[kCtrOverflow], st,
+      acc, in, stk, tsk, &ctr[kCtrOverflow], st,
       [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
         const float4 a = sq.o_tmin[slot];
         const float4 b = sq.d_tmax[slot];
@@ -1361,6 +1465,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const float* partial, uin
 
 size_t stack_lds_bytes(const DevAccel& acc) { return static_cast<size_t>(acc.stack_cap) * 64u * 4u * kWavesPerBlock; }
 size_t ring_lds_bytes() { return static_cast<size_t>(kRingEntries) * 64u * 4u * kWavesPerBlock; }
+// single-level traversals: rings, then each wave's leaf-task area
+size_t sl_lds_bytes() { return ring_lds_bytes() + static_cast<size_t>(kTaskWords) * 4u * kWavesPerBlock; }
 
 }  // namespace
 
@@ -1373,8 +1479,8 @@ int trace_blocks_per_cu(const DevAccel& acc) {
     eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_tl<false>, kBlock, ring_lds_bytes());
     (void)lds;
   } else {
-    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, ring_lds_bytes());
-    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, ring_lds_bytes());
+    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, sl_lds_bytes());
+    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, sl_lds_bytes());
   }
   if (ea != hipSuccess) a = 4;
   if (eb != hipSuccess) b = 4;
@@ -1405,9 +1511,9 @@ void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t
                    hipStream_t stream) {
   if (!acc.two_level) {
     if (stats)
-      hipLaunchKernelGGL(k_extend_sl<true>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, q, h, in, ctr);
+      hipLaunchKernelGGL(k_extend_sl<true>, dim3(grid), dim3(kBlock), sl_lds_bytes(), stream, acc, q, h, in, ctr);
     else
-      hipLaunchKernelGGL(k_extend_sl<false>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, q, h, in,
+      hipLaunchKernelGGL(k_extend_sl<false>, dim3(grid), dim3(kBlock), sl_lds_bytes(), stream, acc, q, h, in,
                          ctr);
     return;
   }
@@ -1432,10 +1538,10 @@ void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn 
                    bool stats, hipStream_t stream) {
   if (!acc.two_level) {
     if (stats)
-      hipLaunchKernelGGL(k_shadow_sl<true>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, sq, radiance,
+      hipLaunchKernelGGL(k_shadow_sl<true>, dim3(grid), dim3(kBlock), sl_lds_bytes(), stream, acc, sq, radiance,
                          in, ctr);
     else
-      hipLaunchKernelGGL(k_shadow_sl<false>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, sq, radiance,
+      hipLaunchKernelGGL(k_shadow_sl<false>, dim3(grid), dim3(kBlock), sl_lds_bytes(), stream, acc, sq, radiance,
                          in, ctr);
     return;
   }
