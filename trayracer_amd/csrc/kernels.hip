@@ -278,9 +278,12 @@ constexpr int32_t kTravDone = static_cast<int32_t>(0x80000000u);
 // The while-while node loops end once at most kSearchBreak of the lanes still walking
 // have no leaf parked, instead of none: the last few searching lanes would otherwise
 // hold the wave for a long tail of iterations in which most lanes idle. They keep their
-// state and go on walking in the next round. On the 1M-triangle soup, 4 to 8 lanes
-// traced 9% more rays per second than 0; 16 was slightly worse.
+// state and go on walking in the next round. With per-lane leaf loops, 4 to 8 lanes
+// traced 9% more rays per second than 0 on the 1M-triangle soup (16 slightly worse);
+// trace_queue, whose leaf phase is spread over the wave and so costs less per round,
+// measured best at 16 (8: -1.5%, 32: -6%).
 constexpr uint32_t kSearchBreak = 8;
+constexpr uint32_t kSearchBreakSpread = 16;
 
 // Lanes of the wave (among the active ones) that hold no parked leaf.
 __device__ __forceinline__ uint32_t searching_lanes(int32_t leaf) {
@@ -385,11 +388,13 @@ __device__ __forceinline__ bool traverse_ww(const DevNode2* __restrict__ nodes, 
 // ("dynamic fetch", Aila & Laine 2009): the wave runs rounds of the speculative
 // while-while loop above, and before each round lanes whose ray has finished take new
 // rays from the queue once at least kRefill of them are idle, so a few long rays no
-// longer hold the whole wave (32 to 40 of 64 measured best on MI355X with the early
-// end of the node loop, 24 and 48 about 1% slower; 64, i.e. no refill, was 18% slower).
+// longer hold the whole wave (40 of 64 measured best on MI355X with the early end of
+// the node loop and the spread leaf phase, 24 and 48 about 1-3% slower; 64, i.e. no
+// refill, was 18% slower with per-lane leaf loops).
 // load(slot, o, d, tmin, tmax) reads a ray;
 // finish(slot, r, hit) consumes a result (hit: a hit was found).
-constexpr uint32_t kRefill = 32;
+constexpr uint32_t kRefill = 40;
+constexpr uint32_t kRefillTwoLevel = 24;  // trace_queue2 (per-lane leaf loops): 24 beat 32 and 40 by ~1%
 
 // Traversal stack of the single-level kernels: the top kRingEntries entries live in an
 // LDS ring (entry k in slot k % kRingEntries, lane-interleaved), older ones are moved
@@ -572,7 +577,7 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
           leaf = node;
           node = pop();
         }
-        if (searching_lanes(leaf) <= kSearchBreak) break;
+        if (searching_lanes(leaf) <= kSearchBreakSpread) break;
       }
     }
     // leaves (the parked one, and `node` when it is a leaf), spread over the wave
@@ -742,7 +747,7 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
   for (;;) {
     const uint64_t idle = __ballot(slot == kNoHit);
     const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
-    if (!drained && n_idle >= kRefill) {
+    if (!drained && n_idle >= kRefillTwoLevel) {
       uint32_t base = 0, got = 0;
       if (seg_fetch_k(in, n_seg, home, seg_done, n_idle, base, got)) {
         const uint32_t rank = prefix_count(idle);
