@@ -287,9 +287,9 @@ trt_status ensure_frame(trt_context* ctx, uint32_t W, uint32_t H, uint32_t spp) 
 
 DevAccel accel_of(const trt_context* ctx) {
   DevAccel a;
-  a.nodes = ctx->d_nodes.as<DevNode2>();
+  a.nodes = ctx->d_nodes.as<DevNode48>();
   a.tris = ctx->d_tris.as<float4>();
-  a.tlas = ctx->two_level ? ctx->d_tlas.as<DevNode2>() : nullptr;
+  a.tlas = ctx->two_level ? ctx->d_tlas.as<DevNode48>() : nullptr;
   a.inst = ctx->two_level ? ctx->d_inst_tlas.as<Instance>() : nullptr;
   a.stack_cap = ctx->stack_cap;
   a.two_level = ctx->two_level ? 1 : 0;
@@ -719,12 +719,16 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   ctx->n_lights = uint32_t(lt.lights.size());
   ctx->total_power = lt.total_power;
 
+  std::vector<DevNode48> nodes48, tlas48;
+  if (!pack_nodes(nodes, nodes48) || !pack_nodes(tlas_nodes, tlas48))
+    return fail(ctx, TRT_UNSUPPORTED, "scene bounds too large for the node layout (infinite coordinates)");
+
   hipStream_t s = ctx->stream;
   HIPC(upload(ctx->d_pos, ctx->pos.data(), ctx->pos.size() * 4, s), "upload positions");
   if (ctx->use_normals) HIPC(upload(ctx->d_nrm, ctx->nrm.data(), ctx->nrm.size() * 4, s), "upload normals");
   HIPC(upload(ctx->d_mat_ids, ctx->mat_ids.data(), ctx->mat_ids.size() * 4, s), "upload material ids");
   HIPC(upload(ctx->d_materials, ctx->materials.data(), ctx->materials.size() * sizeof(Material), s), "upload materials");
-  HIPC(upload(ctx->d_nodes, nodes.data(), nodes.size() * sizeof(DevNode2), s), "upload nodes");
+  HIPC(upload(ctx->d_nodes, nodes48.data(), nodes48.size() * sizeof(DevNode48), s), "upload nodes");
   HIPC(upload(ctx->d_tris, tris.data(), tris.size() * 4, s), "upload triangles");
   HIPC(upload(ctx->d_lights, lt.lights.data(), lt.lights.size() * sizeof(LightTri), s), "upload lights");
   HIPC(upload(ctx->d_alias_p, lt.prob.data(), lt.prob.size() * 4, s), "upload alias");
@@ -732,7 +736,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   if (ctx->two_level) {
     HIPC(upload(ctx->d_inst, inst_orig.data(), inst_orig.size() * sizeof(Instance), s), "upload instances");
     HIPC(upload(ctx->d_inst_tlas, inst_tlas.data(), inst_tlas.size() * sizeof(Instance), s), "upload instances");
-    HIPC(upload(ctx->d_tlas, tlas_nodes.data(), tlas_nodes.size() * sizeof(DevNode2), s), "upload tlas");
+    HIPC(upload(ctx->d_tlas, tlas48.data(), tlas48.size() * sizeof(DevNode48), s), "upload tlas");
   }
   HIPC(hipStreamSynchronize(s), "upload");
 
@@ -752,7 +756,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   ctx->info.stack_entries = uint32_t(cap);
   ctx->info.sah_cost = sah;
   ctx->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  ctx->info.device_bytes = nodes.size() * sizeof(DevNode2) + tris.size() * 4 + tlas_nodes.size() * sizeof(DevNode2);
+  ctx->info.device_bytes = nodes48.size() * sizeof(DevNode48) + tris.size() * 4 + tlas48.size() * sizeof(DevNode48);
   ctx->info.two_level = ctx->two_level ? 1 : 0;
   ctx->info.tlas_depth = uint32_t(tlas_depth);
   ctx->info.references = n_refs;
@@ -828,7 +832,7 @@ trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* o, trt
   out->closest_tris_per_ray = st.extend_rays ? double(c[1]) / st.extend_rays : 0.0;
   out->any_nodes_per_ray = st.shadow_rays ? double(c[2]) / st.shadow_rays : 0.0;
   out->any_tris_per_ray = st.shadow_rays ? double(c[3]) / st.shadow_rays : 0.0;
-  out->node_bytes = sizeof(DevNode2);
+  out->node_bytes = sizeof(DevNode48);
   out->tri_bytes = 48;
   return TRT_OK;
 }

@@ -383,6 +383,41 @@ BuildResult build_binned_sah(const float* prim_box_data, uint32_t n, int bins, i
   return res;
 }
 
+namespace {
+
+// The float nearest to v in the outward direction (down for a lower bound, up for an
+// upper one) whose low byte is `byte`; false if none is finite.
+bool carry_byte(float v, uint32_t byte, bool upper, float& out) {
+  float x = v;
+  for (int k = 0; k <= 512; ++k) {
+    if ((f2u(x) & 0xffu) == byte && std::isfinite(x)) {
+      out = x;
+      return true;
+    }
+    x = std::nextafter(x, upper ? HUGE_VALF : -HUGE_VALF);
+    if (!std::isfinite(x)) return false;
+  }
+  return false;
+}
+
+}  // namespace
+
+bool pack_nodes(const std::vector<DevNode2>& in, std::vector<DevNode48>& out) {
+  out.resize(in.size());
+  for (size_t i = 0; i < in.size(); ++i) {
+    const DevNode2& n = in[i];
+    DevNode48& d = out[i];
+    for (int k = 0; k < 4; ++k) {
+      d.r2[k] = n.r2[k];
+      // row layout: lo.x, hi.x, lo.y, hi.y -> odd entries are upper bounds
+      const bool upper = (k & 1) != 0;
+      const uint32_t b0 = (uint32_t(n.c[0]) >> (8 * k)) & 0xffu, b1 = (uint32_t(n.c[1]) >> (8 * k)) & 0xffu;
+      if (!carry_byte(n.r0[k], b0, upper, d.r0[k]) || !carry_byte(n.r1[k], b1, upper, d.r1[k])) return false;
+    }
+  }
+  return true;
+}
+
 std::vector<DevNode2> to_device_layout(const BuildResult& b) {
   std::vector<DevNode2> out;
   const BuildNode& root = b.nodes[0];

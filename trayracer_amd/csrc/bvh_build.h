@@ -11,6 +11,8 @@
 
 #include <vector>
 
+#include "trt/common.h"
+
 namespace trt {
 
 struct BuildNode {
@@ -54,6 +56,27 @@ struct alignas(16) DevNode2 {
 };
 
 inline int32_t leaf_code(uint32_t first, uint32_t count) { return ~static_cast<int32_t>((first << 3) | count); }
+
+// Traversal layout on the device: 48 bytes, DevNode2's three box rows with the two
+// child references carried in the low byte of the x and y bounds (child0 in row0's four
+// floats, child1 in row1's, least significant byte first), so a node is three 16-byte
+// loads instead of four. Each bound that carries a byte is moved outward (a lower bound
+// down, an upper bound up) to the nearest float whose low byte is that byte, at most
+// 256 ulp, so every stored box contains the exact one and the slab test stays
+// conservative.
+struct alignas(16) DevNode48 {
+  float r0[4], r1[4], r2[4];
+};
+
+// Child reference carried by the low bytes of one row (x0, y0, x1, y1 bounds).
+TRT_FI int32_t node_child_ref(float a, float b, float c, float d) {
+  return static_cast<int32_t>((f2u(a) & 0xffu) | ((f2u(b) & 0xffu) << 8) | ((f2u(c) & 0xffu) << 16) |
+                              (f2u(d) << 24));
+}
+
+// Converts nodes to the 48-byte layout. False if a bound cannot carry its byte (an
+// infinite or maximal bound).
+bool pack_nodes(const std::vector<DevNode2>& in, std::vector<DevNode48>& out);
 
 // Converts a build tree into the device layout (depth-first order, node 0 = root).
 // Returns the device nodes; a root leaf is wrapped in a node with an empty second child.

@@ -35,9 +35,9 @@ struct ShadowQueue {
 
 // Acceleration structure views.
 struct DevAccel {
-  const DevNode2* nodes;    // single-level BVH, or all BLAS nodes for two-level scenes
+  const DevNode48* nodes;   // single-level BVH, or all BLAS nodes for two-level scenes
   const float4* tris;       // reordered traversal triangles: 3 float4 per triangle (w of row 0 = prim id)
-  const DevNode2* tlas;     // top level over instances (nullptr for single-level)
+  const DevNode48* tlas;    // top level over instances (nullptr for single-level)
   const Instance* inst;     // instances (two-level)
   int stack_cap;            // stack entries per lane (tree depth bound)
   int two_level;

@@ -187,20 +187,21 @@ __device__ __forceinline__ bool slab_hw(float lox, float hix, float loy, float h
   return n <= f * kBoxSlack;
 }
 
-__device__ __forceinline__ void load_node(const DevNode2* __restrict__ nodes, int32_t i, float4& r0, float4& r1,
+// One node: three 16-byte loads, child references from the bounds' low bytes.
+__device__ __forceinline__ void load_node(const DevNode48* __restrict__ nodes, int32_t i, float4& r0, float4& r1,
                                           float4& r2, int4& c) {
   const float4* p = reinterpret_cast<const float4*>(nodes + i);
   r0 = p[0];
   r1 = p[1];
   r2 = p[2];
-  c = reinterpret_cast<const int4*>(p)[3];
+  c = make_int4(node_child_ref(r0.x, r0.y, r0.z, r0.w), node_child_ref(r1.x, r1.y, r1.z, r1.w), 0, 0);
 }
 
 // BVH2 traversal of one bottom-level tree (or the single-level scene) for an
 // (object-space) ray. Stack lives in LDS: stk[k * 64] for entry k. Returns true on the
 // first hit when ANY. Updates tbest/pbest/ibest with the (t, inst, prim) order.
 template <bool ANY, bool STATS>
-__device__ __forceinline__ bool traverse_bvh(const DevNode2* __restrict__ nodes, const float4* __restrict__ tris,
+__device__ __forceinline__ bool traverse_bvh(const DevNode48* __restrict__ nodes, const float4* __restrict__ tris,
                                              int32_t root, V3 o, V3 d, float tmin, uint32_t inst, uint32_t* stk,
                                              int sp0, int cap, TraceOut& r, uint32_t* overflow) {
   const RayTri rt = ray_tri_setup(o, d);
@@ -298,7 +299,7 @@ __device__ __forceinline__ uint32_t searching_lanes(int32_t leaf) {
 // every iteration. The result does not depend on the visiting order (closest hit under
 // the (t, inst, prim) order, or any hit), so it equals traverse_bvh's.
 template <bool ANY, bool STATS>
-__device__ __forceinline__ bool traverse_ww(const DevNode2* __restrict__ nodes, const float4* __restrict__ tris, V3 o,
+__device__ __forceinline__ bool traverse_ww(const DevNode48* __restrict__ nodes, const float4* __restrict__ tris, V3 o,
                                             V3 d, float tmin, uint32_t* stk, int cap, TraceOut& r,
                                             uint32_t* overflow) {
   const RayTri rt = ray_tri_setup(o, d);
@@ -492,7 +493,7 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
   const uint32_t lane = lane_id();
   const uint32_t n_seg = seg_counts(in), home = wave_home();
   const int cap = acc.stack_cap;
-  const DevNode2* __restrict__ nodes = acc.nodes;
+  const DevNode48* __restrict__ nodes = acc.nodes;
   const float4* __restrict__ tris = acc.tris;
   uint64_t seg_done = seg_empty(n_seg);
   bool drained = false;
@@ -676,8 +677,8 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
   const uint32_t lane = lane_id();
   const uint32_t n_seg = seg_counts(in), home = wave_home();
   const int cap = acc.stack_cap;
-  const DevNode2* __restrict__ blas = acc.nodes;
-  const DevNode2* __restrict__ tlas = acc.tlas;
+  const DevNode48* __restrict__ blas = acc.nodes;
+  const DevNode48* __restrict__ tlas = acc.tlas;
   const float4* __restrict__ tris = acc.tris;
   uint64_t seg_done = seg_empty(n_seg);
   bool drained = false;
