@@ -73,6 +73,23 @@ def test_trace_soup_exact(rend, presplit):
     assert np.array_equal(rend.trace_any(sr), orc.trace_any(sr))
 
 
+@pytest.mark.parametrize("max_leaf", [1, 7])
+def test_trace_soup_leaf_sizes(rend, max_leaf):
+    """Leaves of 1 and of up to 7 triangles: with 7, a wave's leaf phase holds up to 896
+    triangle tasks, more than one 128-task window, so lanes that do not fit keep their
+    leaves for the next round (trt_trace_closest / _any run the render's own kernels)."""
+    sc = scenes.soup(n_tris=100_000, seed=9, width=64, height=64)
+    rend.UpdateScene(sc, max_leaf=max_leaf, presplit=1.0)
+    orc = _oracle(sc)
+    rays = random_rays(40000, seed=13, lo=-0.3, hi=1.3, tmin=1e-5)
+    hg, ig = rend.trace_closest(rays)
+    hr, ir = orc.trace_closest(rays)
+    _check_hits(hg, ig, hr, ir, f"soup max_leaf={max_leaf} gpu vs oracle")
+    sr = rays.copy()
+    sr[:, 7] = np.random.default_rng(5).uniform(0.001, 0.3, sr.shape[0])
+    assert np.array_equal(rend.trace_any(sr), orc.trace_any(sr))
+
+
 def _compare_paths(pg, pr, label, tol=1e-4):
     denom = np.maximum(np.abs(pr), 1e-6)
     rel = np.abs(pg - pr) / denom

@@ -871,6 +871,10 @@ trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths
   return TRT_OK;
 }
 
+// Test hook of the traversal: the rays go through the render's own Extend (closest) or
+// Shadow (any) kernels, as one depth-0 queue whose slot i is ray i (kSegs segments of
+// ceil(n / kSegs) slots). Occlusion comes back through Shadow's own result path: each
+// ray carries a unit contribution to its own radiance slot, added when unoccluded.
 static trt_status trace_common(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst,
                                uint8_t* occ, bool any) {
   if (!ctx || (n && !rays)) return TRT_INVALID_ARG;
@@ -880,25 +884,63 @@ static trt_status trace_common(trt_context* ctx, const float* rays, uint32_t n, 
   HIPC(ctx->scratch_rays.ensure(size_t(n) * 32), "alloc rays");
   HIPC(ctx->scratch_hits.ensure(size_t(n) * 16), "alloc hits");
   HIPC(ctx->scratch_inst.ensure(size_t(n) * 4), "alloc hits");
-  HIPC(ctx->scratch_occ.ensure(size_t(n) * 4), "alloc hits");
-  HIPC(hipMemcpyAsync(ctx->scratch_rays.p, rays, size_t(n) * 32, hipMemcpyHostToDevice, ctx->stream), "upload rays");
+  HIPC(ctx->scratch_occ.ensure(size_t(n) * 16), "alloc hits");
+  // (o, tmin) rows first, then (d, tmax) rows
+  std::vector<float> soa(size_t(n) * 8);
+  for (uint32_t i = 0; i < n; ++i) {
+    std::memcpy(&soa[size_t(i) * 4], &rays[size_t(i) * 8], 16);
+    std::memcpy(&soa[(size_t(n) + i) * 4], &rays[size_t(i) * 8 + 4], 16);
+  }
+  float4* o_tmin = ctx->scratch_rays.as<float4>();
+  float4* d_tmax = o_tmin + n;
+  HIPC(hipMemcpyAsync(o_tmin, soa.data(), size_t(n) * 32, hipMemcpyHostToDevice, ctx->stream), "upload rays");
   uint32_t* ctr = ctx->counters.as<uint32_t>();
   launch_zero_counters(ctr, ctx->stream);
-  launch_trace(accel_of(ctx), ctx->scratch_rays.as<float4>(), n, ctx->scratch_hits.as<float4>(),
-               ctx->scratch_inst.as<uint32_t>(), ctx->scratch_occ.as<uint32_t>(), any, ctr, ctx->grid_trace,
-               ctx->stream);
+  const uint32_t per = (n + kSegs - 1) / kSegs;
+  std::vector<uint32_t> cnt(size_t(kSegs) * kLine, 0u);
+  for (uint32_t sgm = 0; sgm < kSegs; ++sgm) {
+    const uint64_t first = uint64_t(sgm) * per;
+    cnt[size_t(sgm) * kLine] = first >= n ? 0u : uint32_t(std::min<uint64_t>(per, n - first));
+  }
+  const int kind = any ? kSegShadow : kSegExt, work = any ? kSegWorkShadow : kSegWorkExt;
+  HIPC(hipMemcpyAsync(ctr + seg_ctr(kind, 0), cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, ctx->stream),
+       "upload queue counts");
+  const SegIn in{ctr + seg_ctr(kind, 0), ctr + seg_ctr(work, 0), per};
+  const DevAccel acc = accel_of(ctx);
+  if (any) {
+    std::vector<float> unit(size_t(n) * 4);
+    for (uint32_t i = 0; i < n; ++i) {
+      unit[size_t(i) * 4 + 0] = unit[size_t(i) * 4 + 1] = unit[size_t(i) * 4 + 2] = 1.f;
+      std::memcpy(&unit[size_t(i) * 4 + 3], &i, 4);
+    }
+    float4* contrib = ctx->scratch_hits.as<float4>();
+    float4* result = ctx->scratch_occ.as<float4>();
+    HIPC(hipMemcpyAsync(contrib, unit.data(), size_t(n) * 16, hipMemcpyHostToDevice, ctx->stream), "upload contrib");
+    HIPC(hipMemsetAsync(result, 0, size_t(n) * 16, ctx->stream), "clear result");
+    launch_shadow(acc, ShadowQueue{o_tmin, d_tmax, contrib}, result, in, ctr, ctx->grid_trace, false, ctx->stream);
+  } else {
+    launch_extend(acc, RayQueue{o_tmin, d_tmax, nullptr, nullptr},
+                  HitBuf{ctx->scratch_hits.as<float4>(), ctx->scratch_inst.as<uint32_t>()}, in, ctr, ctx->grid_trace,
+                  false, ctx->stream);
+  }
   HIPC(hipGetLastError(), "trace launch");
   HIPC(hipStreamSynchronize(ctx->stream), "trace");
   uint32_t overflow = 0;
   HIPC(hipMemcpy(&overflow, ctr + kCtrOverflow, 4, hipMemcpyDeviceToHost), "read overflow");
   if (overflow) return fail(ctx, TRT_DEVICE, "traversal stack overflow");
   if (any) {
-    std::vector<uint32_t> o(n);
-    HIPC(hipMemcpy(o.data(), ctx->scratch_occ.p, size_t(n) * 4, hipMemcpyDeviceToHost), "read occlusion");
-    for (uint32_t i = 0; i < n; ++i) occ[i] = uint8_t(o[i]);
+    std::vector<float> r(size_t(n) * 4);
+    HIPC(hipMemcpy(r.data(), ctx->scratch_occ.p, size_t(n) * 16, hipMemcpyDeviceToHost), "read occlusion");
+    for (uint32_t i = 0; i < n; ++i) occ[i] = r[size_t(i) * 4] == 0.f ? 1u : 0u;
   } else {
     HIPC(hipMemcpy(hits, ctx->scratch_hits.p, size_t(n) * 16, hipMemcpyDeviceToHost), "read hits");
-    if (inst) HIPC(hipMemcpy(inst, ctx->scratch_inst.p, size_t(n) * 4, hipMemcpyDeviceToHost), "read hits");
+    if (inst) {
+      if (ctx->two_level) {
+        HIPC(hipMemcpy(inst, ctx->scratch_inst.p, size_t(n) * 4, hipMemcpyDeviceToHost), "read hits");
+      } else {
+        for (uint32_t i = 0; i < n; ++i) inst[i] = kNoInstance;
+      }
+    }
   }
   return TRT_OK;
 }

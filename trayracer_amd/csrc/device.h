@@ -61,7 +61,7 @@ constexpr uint32_t kLine = 32;  // uint32 words per 128-byte line
 //   [kCtrExt + d]    total rays in the extension queue of depth d (d = 0: primary rays),
 //   [kCtrShadow + d] total shadow rays emitted at depth d; both written at the end of a
 //                    frame by k_sum_counts from the per-segment counters
-//   [kCtrTrace]      work cursor of the standalone trace entry points
+//   [kCtrTrace]      unused
 //   [kCtrOverflow]   bit 0: traversal stack overflow; bit 1: queue append failed
 //   [kCtrStats + k]  traversal statistics (node visits, triangle tests: 64-bit pairs)
 //   [seg_ctr(kind, d) + s * kLine] per-segment counters, kinds below
@@ -150,8 +150,5 @@ void launch_shadow_adj(const DevAccel& acc, ShadowQueue sq, const AdjBuffers& ab
 // partial: kGradBlocks * n_mat * 5 floats; out: n_mat * 5 floats (device)
 void launch_grad_reduce(const AdjBuffers& ab, uint32_t mat_lo, uint32_t n_mat, float* partial, float* out,
                         hipStream_t stream);
-// standalone traversal of caller rays (test hooks): rays as (o, tmin, d, tmax)
-void launch_trace(const DevAccel& acc, const float4* rays, uint32_t n, float4* hits, uint32_t* hit_inst,
-                  uint32_t* occluded, bool any, uint32_t* ctr, int grid, hipStream_t stream);
 
 }  // namespace trt

@@ -39,12 +39,6 @@ __device__ __forceinline__ uint32_t prefix_count(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
 }
 
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t* work) {
-  uint32_t base = 0;
-  if (lane_id() == 0) base = atomicAdd(work, 64u);
-  return __shfl(base, 0);
-}
-
 __device__ __forceinline__ void wave_add_stat(unsigned long long* dst, uint32_t v) {
   // sum over the wave, one atomic
   uint32_t s = v;
@@ -1125,8 +1119,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* 
       if (two_level) hinst = h.inst[slot];
     }
     if (b.w >= 0.f) {
-      // this lane is the only writer of radiance[path] in this kernel: read it early
-      const float4 L0 = fb.radiance[meta >> 2];
       PathState ps;
       ps.o = v3(a.x, a.y, a.z);
       ps.d = v3(b.x, b.y, b.z);
@@ -1143,8 +1135,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, const Material* 
       hit.inst = hinst;
       if (DEBUG) shade_debug(sv, rp, ps, hit, depth, out);
       else shade(sv, sampling_mats, rp, ps, hit, depth, out);
-      if (!is_black(out.L)) {
-        float4 L = L0;
+      if (!is_black(out.L)) {  // rare (emitter hits): this lane is the path's only writer here
+        float4 L = fb.radiance[path];
         L.x += out.L.x;
         L.y += out.L.y;
         L.z += out.L.z;
@@ -1206,40 +1198,6 @@ __global__ __launch_bounds__(kBlock) void k_tonemap(const float4* src, uint32_t 
   const float4 c = src[i];
   dst[i] = make_uchar4(static_cast<unsigned char>(tonemap_u8(tp, c.x)), static_cast<unsigned char>(tonemap_u8(tp, c.y)),
                        static_cast<unsigned char>(tonemap_u8(tp, c.z)), 255);
-}
-
-__global__ __launch_bounds__(kBlock) void k_trace(DevAccel acc, const float4* rays, uint32_t n, float4* hits,
-                                                   uint32_t* hit_inst, uint32_t* occluded, uint32_t any,
-                                                   uint32_t* ctr) {
-  extern __shared__ uint32_t lds[];
-  const uint32_t lane = lane_id();
-  uint32_t* stk = lds + (threadIdx.x >> 6) * acc.stack_cap * 64 + lane;
-  for (;;) {
-    const uint32_t base = wave_fetch(&ctr[kCtrTrace]);
-    if (base >= n) break;
-    const uint32_t i = base + lane;
-    if (i < n) {
-      const float4 a = rays[2 * static_cast<size_t>(i)];
-      const float4 b = rays[2 * static_cast<size_t>(i) + 1];
-      TraceOut r;
-      r.t = b.w;
-      r.b1 = 0.f;
-      r.b2 = 0.f;
-      r.prim = kNoHit;
-      r.inst = kNoInstance;
-      r.nodes = 0;
-      r.tests = 0;
-      if (any) {
-        const bool occ =
-            trace_ray<true, false>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
-        occluded[i] = occ ? 1u : 0u;
-      } else {
-        trace_ray<false, false>(acc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, stk, r, &ctr[kCtrOverflow]);
-        hits[i] = make_float4(r.t, r.b1, r.b2, __uint_as_float(r.prim));
-        hit_inst[i] = r.inst;
-      }
-    }
-  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -1478,12 +1436,10 @@ size_t sl_lds_bytes() { return ring_lds_bytes() + static_cast<size_t>(kTaskWords
 
 int trace_blocks_per_cu(const DevAccel& acc) {
   int a = 0, b = 0;
-  const size_t lds = stack_lds_bytes(acc);
   hipError_t ea, eb;
   if (acc.two_level) {
     ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_tl<false>, kBlock, ring_lds_bytes());
     eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_tl<false>, kBlock, ring_lds_bytes());
-    (void)lds;
   } else {
     ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, sl_lds_bytes());
     eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, sl_lds_bytes());
@@ -1564,12 +1520,6 @@ void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffer
   const uint32_t npix = rp.width * rp.height;
   hipLaunchKernelGGL(k_finalize, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, rp, radiance, fb,
                      reset);
-}
-
-void launch_trace(const DevAccel& acc, const float4* rays, uint32_t n, float4* hits, uint32_t* hit_inst,
-                  uint32_t* occluded, bool any, uint32_t* ctr, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k_trace, dim3(grid), dim3(kBlock), stack_lds_bytes(acc), stream, acc, rays, n, hits, hit_inst,
-                     occluded, any ? 1u : 0u, ctr);
 }
 
 
