@@ -294,7 +294,8 @@ trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths
 /* Ray queries on the built scene (replace optixTrace closest / terminate-on-first-hit,
  * Tracer/Optix/GPU/optix.cu:264-265, 333-335). rays: 8 floats per ray
  * (ox, oy, oz, tmin, dx, dy, dz, tmax). hits: 4 floats per ray (t, b1, b2, prim as uint
- * bits; prim = 0xffffffff on a miss); inst: instance per ray (0xffffffff single-level). */
+ * bits; prim = 0xffffffff on a miss); inst: instance per ray (0xffffffff single-level).
+ * Both run the renderer's own Extend / Shadow traversal kernels over the rays. */
 trt_status trt_trace_closest(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst);
 trt_status trt_trace_any(trt_context* ctx, const float* rays, uint32_t n, uint8_t* occluded);
 
