@@ -273,10 +273,10 @@ constexpr int32_t kTravDone = static_cast<int32_t>(0x80000000u);
 // The while-while node loops end once at most kSearchBreak of the lanes still walking
 // have no leaf parked, instead of none: the last few searching lanes would otherwise
 // hold the wave for a long tail of iterations in which most lanes idle. They keep their
-// state and go on walking in the next round. With per-lane leaf loops, 4 to 8 lanes
-// traced 9% more rays per second than 0 on the 1M-triangle soup (16 slightly worse);
-// trace_queue, whose leaf phase is spread over the wave and so costs less per round,
-// measured best at 16 (8: -1.5%, 32: -6%).
+// state and go on walking in the next round. With per-lane leaf loops (traverse_ww), 4
+// to 8 lanes traced 9% more rays per second than 0 on the 1M-triangle soup (16 slightly
+// worse); the queue loops, whose leaf phase is spread over the wave and so costs less
+// per round, measured best at 16 (soup: 8 -1.5%, 32 -6%; instanced: 8 -2%).
 constexpr uint32_t kSearchBreak = 8;
 constexpr uint32_t kSearchBreakSpread = 16;
 
@@ -389,7 +389,6 @@ __device__ __forceinline__ bool traverse_ww(const DevNode48* __restrict__ nodes,
 // load(slot, o, d, tmin, tmax) reads a ray;
 // finish(slot, r, hit) consumes a result (hit: a hit was found).
 constexpr uint32_t kRefill = 40;
-constexpr uint32_t kRefillTwoLevel = 24;  // trace_queue2 (per-lane leaf loops): 24 beat 32 and 40 by ~1%
 
 // Traversal stack of the single-level kernels: the top kRingEntries entries live in an
 // LDS ring (entry k in slot k % kRingEntries, lane-interleaved), older ones are moved
@@ -481,6 +480,86 @@ __device__ __forceinline__ float key_t(unsigned long long k) {
   return __uint_as_float(u);
 }
 
+// One leaf phase: lane l contributes the tasks of ca triangles from fa and cb from fb,
+// tested with its ray (object-space origin o, ray_tri_setup's packed axes and shear,
+// tmin, tmax). A task hit counts if it beats the lane's best (t = tmax, prim0, and for
+// TWO, instance inst0, with the lane's current instance cur). Returns whether the
+// lane's tasks were in this round's window (false: it keeps its leaves for the next
+// round); `better`: a task hit counted, the smallest (t, prim) key of those is kf, and
+// its barycentrics are in tk.bary[lane].
+template <bool STATS, bool TWO>
+__device__ __forceinline__ bool leaf_phase(const LeafTasks& tk, const float4* __restrict__ tris, uint32_t lane,
+                                           uint32_t fa, uint32_t ca, uint32_t fb, uint32_t cb, V3 o, int32_t kxyz,
+                                           V3 shear, float tmin, float tmax, uint32_t prim0, uint32_t inst0,
+                                           uint32_t cur, bool& better, unsigned long long& kf, TraceOut& stats) {
+  const uint32_t n_tasks = ca + cb;  // at most 14
+  uint32_t off = 0, total = 0;       // exclusive prefix and sum of n_tasks over the wave
+  for (uint32_t bit = 0; bit < 4; ++bit) {
+    const uint64_t m = __ballot((n_tasks >> bit) & 1u);
+    off += prefix_count(m) << bit;
+    total += static_cast<uint32_t>(__popcll(m)) << bit;
+  }
+  better = false;
+  kf = 0;
+  if (total == 0) return true;
+  // one window of kTaskWin tasks per round: lanes whose tasks do not fit keep their
+  // leaves for the next round (the first lane with tasks always fits)
+  const bool fits = off + n_tasks <= kTaskWin;
+  const uint64_t late = __ballot(!fits);
+  const uint32_t m = late != 0 ? __shfl(off, static_cast<int>(__builtin_ctzll(late))) : total;
+  // TWO: every hit up to tmax is a candidate, the owner's order is checked per hit
+  const unsigned long long key0 = hit_key(tmax, TWO ? kNoHit : prim0);
+  tk.key[lane] = key0;
+  if (fits) {
+    for (uint32_t k = 0; k < n_tasks; ++k) {
+      tk.tri[off + k] = k < ca ? fa + k : fb + (k - ca);
+      tk.own[off + k] = static_cast<uint8_t>(lane);
+    }
+  }
+  for (uint32_t s0 = 0; s0 < m; s0 += 64) {
+    const uint32_t j = s0 + lane;
+    const bool act = j < m;
+    const uint32_t ti = act ? tk.tri[j] : 0u;
+    const int own = act ? static_cast<int>(tk.own[j]) : static_cast<int>(lane);
+    RayTri ort;
+    ort.o = v3(__shfl(o.x, own), __shfl(o.y, own), __shfl(o.z, own));
+    const int kk = __shfl(kxyz, own);
+    ort.kx = kk & 3;
+    ort.ky = (kk >> 2) & 3;
+    ort.kz = kk >> 4;
+    ort.sx = __shfl(shear.x, own);
+    ort.sy = __shfl(shear.y, own);
+    ort.sz = __shfl(shear.z, own);
+    const float otmin = __shfl(tmin, own), otmax = __shfl(tmax, own);
+    uint32_t oprim = 0, oinst = 0, ocur = 0;
+    if (TWO) {
+      oprim = __shfl(prim0, own);
+      oinst = __shfl(inst0, own);
+      ocur = __shfl(cur, own);
+    }
+    if (act) {
+      const float4* tp = tris + static_cast<size_t>(ti) * 3u;
+      const float4 a = tp[0], b = tp[1], cc = tp[2];
+      if (STATS) stats.tests++;
+      TriHit th;
+      const uint32_t prim = __float_as_uint(a.w);
+      if (intersect_triangle(ort, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), otmin, otmax, th) &&
+          (!TWO || th.t < otmax || ocur < oinst || (ocur == oinst && prim < oprim))) {
+        const unsigned long long key = hit_key(th.t, prim);
+        atomicMin(&tk.key[own], key);
+        if (tk.key[own] == key) tk.bary[own] = make_float2(th.b1, th.b2);
+      }
+    }
+  }
+  if (fits && n_tasks != 0) {
+    kf = tk.key[lane];
+    better = kf < key0;
+  }
+  return fits;
+}
+
+__device__ __forceinline__ int32_t pack_axes(const RayTri& t) { return t.kx | (t.ky << 2) | (t.kz << 4); }
+
 template <bool ANY, bool STATS, class Load, class Finish>
 __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in, uint32_t* stk, uint32_t* tsk,
                                             uint32_t* overflow, TraceOut& stats, Load load, Finish finish) {
@@ -523,7 +602,7 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
           float tmax;
           load(slot, o, d, tmin, tmax);
           const RayTri rt = ray_tri_setup(o, d);
-          kxyz = rt.kx | (rt.ky << 2) | (rt.kz << 4);
+          kxyz = pack_axes(rt);
           shear = v3(rt.sx, rt.sy, rt.sz);
           rb = ray_box_setup(o, d);
           r.t = tmax;
@@ -580,65 +659,19 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
     const bool node_leaf = node < 0 && node != kTravDone;
     const uint32_t ca = leaf != 0 ? (~static_cast<uint32_t>(leaf) & 7u) : 0u;
     const uint32_t cb = node_leaf ? (~static_cast<uint32_t>(node) & 7u) : 0u;
-    const uint32_t fa = ~static_cast<uint32_t>(leaf) >> 3, fb = ~static_cast<uint32_t>(node) >> 3;
-    const uint32_t n_tasks = ca + cb;  // at most 14
-    uint32_t off = 0, total = 0;       // exclusive prefix and sum of n_tasks over the wave
-    for (uint32_t bit = 0; bit < 4; ++bit) {
-      const uint64_t m = __ballot((n_tasks >> bit) & 1u);
-      off += prefix_count(m) << bit;
-      total += static_cast<uint32_t>(__popcll(m)) << bit;
-    }
-    if (total != 0) {
-      // one window of kTaskWin tasks per round: lanes whose tasks do not fit keep their
-      // leaves for the next round (the first lane with tasks always fits)
-      const bool fits = off + n_tasks <= kTaskWin;
-      const uint64_t late = __ballot(!fits);
-      const uint32_t m = late != 0 ? __shfl(off, static_cast<int>(__builtin_ctzll(late))) : total;
-      const unsigned long long key0 = hit_key(r.t, r.prim);
-      tk.key[lane] = key0;
-      if (fits) {
-        for (uint32_t k = 0; k < n_tasks; ++k) {
-          tk.tri[off + k] = k < ca ? fa + k : fb + (k - ca);
-          tk.own[off + k] = static_cast<uint8_t>(lane);
-        }
+    bool better;
+    unsigned long long kf;
+    if (leaf_phase<STATS, false>(tk, tris, lane, ~static_cast<uint32_t>(leaf) >> 3, ca, ~static_cast<uint32_t>(node) >> 3,
+                                 cb, rb.o, kxyz, shear, tmin, r.t, r.prim, kNoInstance, kNoInstance, better, kf,
+                                 stats) &&
+        ca + cb != 0) {
+      if (better) {  // the barycentrics stay in tk.bary[lane] until the ray finishes
+        r.t = key_t(kf);
+        r.prim = static_cast<uint32_t>(kf);
+        hit = ANY;
       }
-      for (uint32_t s0 = 0; s0 < m; s0 += 64) {
-        const uint32_t j = s0 + lane;
-        const bool act = j < m;
-        const uint32_t ti = act ? tk.tri[j] : 0u;
-        const int own = act ? static_cast<int>(tk.own[j]) : static_cast<int>(lane);
-        RayTri ort;
-        ort.o = v3(__shfl(rb.o.x, own), __shfl(rb.o.y, own), __shfl(rb.o.z, own));
-        const int kk = __shfl(kxyz, own);
-        ort.kx = kk & 3;
-        ort.ky = (kk >> 2) & 3;
-        ort.kz = kk >> 4;
-        ort.sx = __shfl(shear.x, own);
-        ort.sy = __shfl(shear.y, own);
-        ort.sz = __shfl(shear.z, own);
-        const float otmin = __shfl(tmin, own), otmax = __shfl(r.t, own);
-        if (act) {
-          const float4* tp = tris + static_cast<size_t>(ti) * 3u;
-          const float4 a = tp[0], b = tp[1], cc = tp[2];
-          if (STATS) stats.tests++;
-          TriHit th;
-          if (intersect_triangle(ort, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), otmin, otmax, th)) {
-            const unsigned long long key = hit_key(th.t, __float_as_uint(a.w));
-            atomicMin(&tk.key[own], key);
-            if (tk.key[own] == key) tk.bary[own] = make_float2(th.b1, th.b2);
-          }
-        }
-      }
-      if (fits && n_tasks != 0) {
-        const unsigned long long kf = tk.key[lane];
-        if (kf < key0) {  // the barycentrics stay in tk.bary[lane] until the ray finishes
-          r.t = key_t(kf);
-          r.prim = static_cast<uint32_t>(kf);
-          hit = ANY;
-        }
-        leaf = 0;
-        if (node_leaf) node = pop();
-      }
+      leaf = 0;
+      if (node_leaf) node = pop();
     }
     if (slot != kNoHit && (hit || (node == kTravDone && leaf == 0))) {
       r.b1 = r.b2 = 0.f;
@@ -656,8 +689,9 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
 }
 
 // Two-level counterpart of trace_queue (instanced scenes): the same speculative
-// while-while loop with lane refill and two-tier stack, walking the top-level tree over
-// instance boxes and, inside an instance, its mesh's tree with the ray in object space.
+// while-while loop with lane refill, two-tier stack and wave-wide leaf phase, walking the
+// top-level tree over instance boxes and, inside an instance, its mesh's tree with the
+// ray in object space (leaf tasks use the owner's object-space ray).
 // Entering an instance pushes what is left of the top-level leaf and an exit marker;
 // popping the marker returns the ray to world space. A lane with a parked (not yet
 // intersected) leaf never leaves its instance: it waits at the marker until the leaf
@@ -666,8 +700,8 @@ constexpr int32_t kTravExit = 0x7fffffff;                               // leave
 constexpr int32_t kTravBlocked = static_cast<int32_t>(0x80000001u);     // waits at the marker with a parked leaf
 
 template <bool ANY, bool STATS, class Load, class Finish>
-__device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& in, uint32_t* stk, uint32_t* overflow,
-                                             TraceOut& stats, Load load, Finish finish) {
+__device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& in, uint32_t* stk, uint32_t* tsk,
+                                             uint32_t* overflow, TraceOut& stats, Load load, Finish finish) {
   const uint32_t lane = lane_id();
   const uint32_t n_seg = seg_counts(in), home = wave_home();
   const int cap = acc.stack_cap;
@@ -678,8 +712,9 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
   bool drained = false;
   uint32_t slot = kNoHit;
   V3 ow = v3(0.f), dw = v3(0.f);
-  RayBox rbw{}, rb{};
-  RayTri rt{};
+  RayBox rb{};  // in the current space (world outside instances)
+  int32_t kxyz = 0;    // ray_tri_setup's axis permutation in the current instance, 2 bits per axis
+  V3 shear = v3(0.f);  // and its shear constants
   float tmin = 0.f;
   TraceOut r;
   r.t = 0.f;
@@ -714,7 +749,7 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
         return kTravBlocked;
       }
       in_blas = false;
-      rb = rbw;
+      rb = ray_box_setup(ow, dw);  // back in world space (recomputed: fewer live registers)
     }
   };
   // Top-level leaf (instances [first, first + count)): enter the first instance.
@@ -727,7 +762,9 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
     const V3 o = xform_point(ins.w2o, ow);
     const V3 d = xform_vector(ins.w2o, dw);
     rb = ray_box_setup(o, d);
-    rt = ray_tri_setup(o, d);
+    const RayTri rt = ray_tri_setup(o, d);
+    kxyz = pack_axes(rt);
+    shear = v3(rt.sx, rt.sy, rt.sz);
     cur_id = ins.id;
     in_blas = true;
     if (STATS) stats.nodes++;
@@ -739,10 +776,11 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
     while (!in_blas && n < 0 && n != kTravDone && n != kTravBlocked) n = enter(n);
     return n;
   };
+  const LeafTasks tk = leaf_tasks(tsk);
   for (;;) {
     const uint64_t idle = __ballot(slot == kNoHit);
     const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
-    if (!drained && n_idle >= kRefillTwoLevel) {
+    if (!drained && n_idle >= kRefill) {
       uint32_t base = 0, got = 0;
       if (seg_fetch_k(in, n_seg, home, seg_done, n_idle, base, got)) {
         const uint32_t rank = prefix_count(idle);
@@ -750,11 +788,9 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
           slot = base + rank;
           float tmax;
           load(slot, ow, dw, tmin, tmax);
-          rbw = ray_box_setup(ow, dw);
-          rb = rbw;
+          rb = ray_box_setup(ow, dw);
           in_blas = false;
           r.t = tmax;
-          r.b1 = r.b2 = 0.f;
           r.prim = kNoHit;
           r.inst = kNoInstance;
           st.sp = 0;
@@ -767,75 +803,69 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
       }
     }
     if (!__any(slot != kNoHit)) break;
-    if (slot == kNoHit) continue;
+    if (slot != kNoHit) {
+      // inner nodes of either level
+      while (node >= 0) {
+        float4 r0, r1, r2;
+        int4 c;
+        load_node(in_blas ? blas : tlas, node, r0, r1, r2, c);
+        if (STATS) stats.nodes++;
+        float t0, t1;
+        const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+        const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+        if (h0 && h1) {
+          int32_t nearc = c.x, farc = c.y;
+          if (t1 < t0) {
+            nearc = c.y;
+            farc = c.x;
+          }
+          push(static_cast<uint32_t>(farc));
+          node = nearc;
+        } else if (h0) {
+          node = c.x;
+        } else if (h1) {
+          node = c.y;
+        } else {
+          node = pop();
+        }
+        node = settle(node);
+        if (in_blas && node < 0 && node != kTravDone && node != kTravBlocked && leaf == 0) {
+          leaf = node;
+          node = settle(pop());
+        }
+        if (searching_lanes(leaf) <= kSearchBreakSpread) break;
+      }
+    }
+    // mesh leaves (the parked one, and `node` when it is one; both in the current
+    // instance, which a lane with a parked leaf does not leave), spread over the wave.
+    // Task hits that beat the ray's best under the (t, inst, prim) order compete; the
+    // phase's (t, prim) minimum of those becomes the best.
     bool hit = false;
-    // inner nodes of either level
-    while (node >= 0) {
-      float4 r0, r1, r2;
-      int4 c;
-      load_node(in_blas ? blas : tlas, node, r0, r1, r2, c);
-      if (STATS) stats.nodes++;
-      float t0, t1;
-      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
-      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
-      if (h0 && h1) {
-        int32_t nearc = c.x, farc = c.y;
-        if (t1 < t0) {
-          nearc = c.y;
-          farc = c.x;
-        }
-        push(static_cast<uint32_t>(farc));
-        node = nearc;
-      } else if (h0) {
-        node = c.x;
-      } else if (h1) {
-        node = c.y;
-      } else {
-        node = pop();
+    const bool node_leaf = in_blas && node < 0 && node != kTravDone && node != kTravBlocked;
+    const uint32_t ca = leaf != 0 ? (~static_cast<uint32_t>(leaf) & 7u) : 0u;
+    const uint32_t cb = node_leaf ? (~static_cast<uint32_t>(node) & 7u) : 0u;
+    bool better;
+    unsigned long long kf;
+    if (leaf_phase<STATS, true>(tk, tris, lane, ~static_cast<uint32_t>(leaf) >> 3, ca, ~static_cast<uint32_t>(node) >> 3,
+                                cb, rb.o, kxyz, shear, tmin, r.t, r.prim, r.inst, cur_id, better, kf, stats) &&
+        ca + cb != 0) {
+      if (better) {  // the barycentrics stay in tk.bary[lane] until the ray finishes
+        r.t = key_t(kf);
+        r.prim = static_cast<uint32_t>(kf);
+        r.inst = cur_id;
+        hit = ANY;
       }
-      node = settle(node);
-      if (in_blas && node < 0 && node != kTravDone && node != kTravBlocked && leaf == 0) {
-        leaf = node;
-        node = settle(pop());
-      }
-      if (searching_lanes(leaf) <= kSearchBreak) break;
+      leaf = 0;
+      if (node_leaf) node = settle(pop());
+      if (node == kTravBlocked) node = settle(pop());  // the leaf is done: leave the instance
     }
-    // mesh leaves (in the current instance's space)
-    while (leaf != 0) {
-      const uint32_t code = ~static_cast<uint32_t>(leaf);
-      const uint32_t first = code >> 3, cnt = code & 7u;
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const float4* tp = tris + static_cast<size_t>(first + k) * 3u;
-        const float4 a = tp[0], b = tp[1], cc = tp[2];
-        if (STATS) stats.tests++;
-        TriHit th;
-        if (intersect_triangle(rt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), tmin, r.t, th)) {
-          const uint32_t prim = __float_as_uint(a.w);
-          if (th.t < r.t || cur_id < r.inst || (cur_id == r.inst && prim < r.prim)) {
-            r.t = th.t;
-            r.b1 = th.b1;
-            r.b2 = th.b2;
-            r.prim = prim;
-            r.inst = cur_id;
-          }
-          if (ANY) {
-            hit = true;
-            break;
-          }
-        }
+    if (slot != kNoHit && (hit || (node == kTravDone && leaf == 0))) {
+      r.b1 = r.b2 = 0.f;
+      if (r.prim != kNoHit) {
+        const float2 bb = tk.bary[lane];
+        r.b1 = bb.x;
+        r.b2 = bb.y;
       }
-      if (ANY && hit) break;
-      if (node == kTravBlocked) {
-        leaf = 0;
-        node = settle(pop());
-      } else if (in_blas && node < 0 && node != kTravDone) {
-        leaf = node;
-        node = settle(pop());
-      } else {
-        leaf = 0;
-      }
-    }
-    if (hit || (node == kTravDone && leaf == 0)) {
       finish(slot, r, hit || r.prim != kNoHit);
       slot = kNoHit;
       node = kTravDone;
@@ -961,9 +991,11 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams rp, uint32_t n_p
 }
 
 
-// Single-level scenes: Extend and Shadow through trace_queue.
+// Single-level scenes: Extend and Shadow through trace_queue. Both are held to 72
+// VGPRs, 7 waves per SIMD (the allocator otherwise settles just above, at 6 waves,
+// which measured 5% slower).
 template <bool STATS>
-__global__ __launch_bounds__(kBlock) void k_extend_sl(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_extend_sl(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
   uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
   uint32_t* tsk = lds + kWavesPerBlock * kRingEntries * 64 + (threadIdx.x >> 6) * kTaskWords;
@@ -990,7 +1022,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_sl(DevAccel acc, RayQueue q, 
 }
 
 template <bool STATS>
-__global__ __launch_bounds__(kBlock) void k_shadow_sl(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow_sl(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
                                                        uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
   uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
@@ -1025,16 +1057,18 @@ __global__ __launch_bounds__(kBlock) void k_shadow_sl(DevAccel acc, ShadowQueue 
   }
 }
 
-// Two-level scenes: Extend and Shadow through trace_queue2.
+// Two-level scenes: Extend and Shadow through trace_queue2, held to 80 VGPRs, 6 waves
+// per SIMD (a few spilled dwords; 5 waves at the allocator's 86 measured 5% slower).
 template <bool STATS>
-__global__ __launch_bounds__(kBlock) void k_extend_tl(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_extend_tl(DevAccel acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
   uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
+  uint32_t* tsk = lds + kWavesPerBlock * kRingEntries * 64 + (threadIdx.x >> 6) * kTaskWords;
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
   trace_queue2<false, STATS>(
-      acc, in, stk, &ctr[kCtrOverflow], st,
+      acc, in, stk, tsk, &ctr[kCtrOverflow], st,
       [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
         const float4 a = q.o_tmin[slot];
         const float4 b = q.d_tmax[slot];
@@ -1054,15 +1088,16 @@ __global__ __launch_bounds__(kBlock) void k_extend_tl(DevAccel acc, RayQueue q, 
 }
 
 template <bool STATS>
-__global__ __launch_bounds__(kBlock) void k_shadow_tl(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow_tl(DevAccel acc, ShadowQueue sq, float4* radiance, SegIn in,
                                                        uint32_t* ctr) {
   extern __shared__ uint32_t lds[];
   uint32_t* stk = lds + (threadIdx.x >> 6) * kRingEntries * 64 + lane_id();
+  uint32_t* tsk = lds + kWavesPerBlock * kRingEntries * 64 + (threadIdx.x >> 6) * kTaskWords;
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
   trace_queue2<true, STATS>(
-      acc, in, stk, &ctr[kCtrOverflow], st,
+      acc, in, stk, tsk, &ctr[kCtrOverflow], st,
       [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
         const float4 a = sq.o_tmin[slot];
         const float4 b = sq.d_tmax[slot];
@@ -1429,8 +1464,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_final(const float* partial, uin
 
 size_t stack_lds_bytes(const DevAccel& acc) { return static_cast<size_t>(acc.stack_cap) * 64u * 4u * kWavesPerBlock; }
 size_t ring_lds_bytes() { return static_cast<size_t>(kRingEntries) * 64u * 4u * kWavesPerBlock; }
-// single-level traversals: rings, then each wave's leaf-task area
-size_t sl_lds_bytes() { return ring_lds_bytes() + static_cast<size_t>(kTaskWords) * 4u * kWavesPerBlock; }
+// queue traversals: rings, then each wave's leaf-task area
+size_t trace_lds_bytes() { return ring_lds_bytes() + static_cast<size_t>(kTaskWords) * 4u * kWavesPerBlock; }
 
 }  // namespace
 
@@ -1438,11 +1473,11 @@ int trace_blocks_per_cu(const DevAccel& acc) {
   int a = 0, b = 0;
   hipError_t ea, eb;
   if (acc.two_level) {
-    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_tl<false>, kBlock, ring_lds_bytes());
-    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_tl<false>, kBlock, ring_lds_bytes());
+    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_tl<false>, kBlock, trace_lds_bytes());
+    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_tl<false>, kBlock, trace_lds_bytes());
   } else {
-    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, sl_lds_bytes());
-    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, sl_lds_bytes());
+    ea = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_sl<false>, kBlock, trace_lds_bytes());
+    eb = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_sl<false>, kBlock, trace_lds_bytes());
   }
   if (ea != hipSuccess) a = 4;
   if (eb != hipSuccess) b = 4;
@@ -1473,16 +1508,16 @@ void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t
                    hipStream_t stream) {
   if (!acc.two_level) {
     if (stats)
-      hipLaunchKernelGGL(k_extend_sl<true>, dim3(grid), dim3(kBlock), sl_lds_bytes(), stream, acc, q, h, in, ctr);
+      hipLaunchKernelGGL(k_extend_sl<true>, dim3(grid), dim3(kBlock), trace_lds_bytes(), stream, acc, q, h, in, ctr);
     else
-      hipLaunchKernelGGL(k_extend_sl<false>, dim3(grid), dim3(kBlock), sl_lds_bytes(), stream, acc, q, h, in,
+      hipLaunchKernelGGL(k_extend_sl<false>, dim3(grid), dim3(kBlock), trace_lds_bytes(), stream, acc, q, h, in,
                          ctr);
     return;
   }
   if (stats)
-    hipLaunchKernelGGL(k_extend_tl<true>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, q, h, in, ctr);
+    hipLaunchKernelGGL(k_extend_tl<true>, dim3(grid), dim3(kBlock), trace_lds_bytes(), stream, acc, q, h, in, ctr);
   else
-    hipLaunchKernelGGL(k_extend_tl<false>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, q, h, in, ctr);
+    hipLaunchKernelGGL(k_extend_tl<false>, dim3(grid), dim3(kBlock), trace_lds_bytes(), stream, acc, q, h, in, ctr);
 }
 
 void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
@@ -1500,18 +1535,18 @@ void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn 
                    bool stats, hipStream_t stream) {
   if (!acc.two_level) {
     if (stats)
-      hipLaunchKernelGGL(k_shadow_sl<true>, dim3(grid), dim3(kBlock), sl_lds_bytes(), stream, acc, sq, radiance,
+      hipLaunchKernelGGL(k_shadow_sl<true>, dim3(grid), dim3(kBlock), trace_lds_bytes(), stream, acc, sq, radiance,
                          in, ctr);
     else
-      hipLaunchKernelGGL(k_shadow_sl<false>, dim3(grid), dim3(kBlock), sl_lds_bytes(), stream, acc, sq, radiance,
+      hipLaunchKernelGGL(k_shadow_sl<false>, dim3(grid), dim3(kBlock), trace_lds_bytes(), stream, acc, sq, radiance,
                          in, ctr);
     return;
   }
   if (stats)
-    hipLaunchKernelGGL(k_shadow_tl<true>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, sq, radiance, in,
+    hipLaunchKernelGGL(k_shadow_tl<true>, dim3(grid), dim3(kBlock), trace_lds_bytes(), stream, acc, sq, radiance, in,
                        ctr);
   else
-    hipLaunchKernelGGL(k_shadow_tl<false>, dim3(grid), dim3(kBlock), ring_lds_bytes(), stream, acc, sq, radiance, in,
+    hipLaunchKernelGGL(k_shadow_tl<false>, dim3(grid), dim3(kBlock), trace_lds_bytes(), stream, acc, sq, radiance, in,
                        ctr);
 }
 
