@@ -61,14 +61,12 @@ constexpr uint32_t kLine = 32;  // uint32 words per 128-byte line
 //   [kCtrExt + d]    total rays in the extension queue of depth d (d = 0: primary rays),
 //   [kCtrShadow + d] total shadow rays emitted at depth d; both written at the end of a
 //                    frame by k_sum_counts from the per-segment counters
-//   [kCtrTrace]      unused
 //   [kCtrOverflow]   bit 0: traversal stack overflow; bit 1: queue append failed
 //   [kCtrStats + k]  traversal statistics (node visits, triangle tests: 64-bit pairs)
 //   [seg_ctr(kind, d) + s * kLine] per-segment counters, kinds below
 constexpr int kMaxDepth = 32;
 constexpr int kCtrExt = 0;
 constexpr int kCtrShadow = 32;
-constexpr int kCtrTrace = 160;
 constexpr int kCtrOverflow = 161;
 constexpr int kCtrStats = 164;  // 4 x uint64: closest nodes, closest tris, any nodes, any tris
 constexpr int kCtrSegBase = 256;
