@@ -42,7 +42,7 @@ int main(int argc, char** argv) {
       box[size_t(i) * 6 + 3 + k] = (i % 7 == 0) ? 0.f : c + e;
     }
   }
-  const trt::BuildResult b = trt::build_binned_sah(box.data(), n, 32, 4, 2);
+  const trt::BuildResult b = trt::build_binned_sah(box.data(), n, 32, 4, 2, trt::kTravCostTriangles);
   const std::vector<trt::DevNode2> dn = trt::to_device_layout(b);
   std::vector<trt::DevNode48> pk;
   if (!trt::pack_nodes(dn, pk)) return fail("pack refused finite boxes", 0);

@@ -88,7 +88,7 @@ int main(int argc, char** argv) {
       if (!in) return fail("point not covered", i);
     }
   }
-  trt::BuildResult br = trt::build_binned_sah(box.data(), uint32_t(refp.size()), 32, 4, 4);
+  trt::BuildResult br = trt::build_binned_sah(box.data(), uint32_t(refp.size()), 32, 4, 4, trt::kTravCostTriangles);
   std::vector<int> seen(refp.size(), 0);
   for (const trt::BuildNode& nd : br.nodes) {
     if (nd.count == 0) continue;

@@ -213,7 +213,7 @@ BuildResult build_mesh(const float* pos, uint32_t first, uint32_t n, const trt_b
     ref_prim.resize(n);
     for (uint32_t i = 0; i < n; ++i) ref_prim[i] = i;
   }
-  return build_binned_sah(box.data(), uint32_t(ref_prim.size()), o.bins, o.max_leaf, threads);
+  return build_binned_sah(box.data(), uint32_t(ref_prim.size()), o.bins, o.max_leaf, threads, kTravCostTriangles);
 }
 
 // Appends a mesh BVH in device layout: node indices offset by node_base, leaf
@@ -698,7 +698,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
         ibox[size_t(i) * 6 + 3 + k] = hi[k] + pad;
       }
     }
-    BuildResult tb = build_binned_sah(ibox.data(), ni, o.bins, o.max_leaf, threads);
+    BuildResult tb = build_binned_sah(ibox.data(), ni, o.bins, o.max_leaf, threads, kTravCostInstances);
     tlas_nodes = to_device_layout(tb);
     inst_tlas.resize(ni);
     for (uint32_t k = 0; k < ni; ++k) inst_tlas[k] = inst_orig[tb.prim_order[k]];

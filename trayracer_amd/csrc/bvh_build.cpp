@@ -44,6 +44,7 @@ struct Ctx {
   std::vector<uint32_t>* order;
   int bins;
   int max_leaf;
+  double trav_cost;
   std::atomic<int> threads_left;
 };
 
@@ -135,7 +136,8 @@ int build_rec(Ctx& c, uint32_t begin, uint32_t end, SubTree& st, int level) {
   }
   const double parent_area = bb.area();
   const double leaf_cost = double(n);
-  const double split_cost = parent_area > 0 ? 1.0 + best_cost / parent_area : std::numeric_limits<double>::infinity();
+  const double split_cost =
+      parent_area > 0 ? c.trav_cost + best_cost / parent_area : std::numeric_limits<double>::infinity();
   if (n <= uint32_t(c.max_leaf) && (best_axis < 0 || leaf_cost <= split_cost)) return make_leaf(c, begin, end, bb, st);
 
   uint32_t mid;
@@ -352,7 +354,8 @@ void presplit_triangles(const float* pos, uint32_t n, float beta, std::vector<fl
   }
 }
 
-BuildResult build_binned_sah(const float* prim_box_data, uint32_t n, int bins, int max_leaf, int n_threads) {
+BuildResult build_binned_sah(const float* prim_box_data, uint32_t n, int bins, int max_leaf, int n_threads,
+                             double trav_cost) {
   BuildResult res;
   res.prim_order.resize(n);
   for (uint32_t i = 0; i < n; ++i) res.prim_order[i] = i;
@@ -374,6 +377,7 @@ BuildResult build_binned_sah(const float* prim_box_data, uint32_t n, int bins, i
   c.order = &res.prim_order;
   c.bins = std::max(2, bins);
   c.max_leaf = std::max(1, std::min(7, max_leaf));
+  c.trav_cost = trav_cost;
   c.threads_left = std::max(0, n_threads - 1);
   SubTree st;
   st.nodes.reserve(size_t(n) * 2 / std::max(1, c.max_leaf / 2) + 16);
