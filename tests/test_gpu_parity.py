@@ -73,6 +73,54 @@ def test_trace_soup_exact(rend, presplit):
     assert np.array_equal(rend.trace_any(sr), orc.trace_any(sr))
 
 
+def _axis_rays(n, seed):
+    """Degenerate rays for the slab test: directions along +-x, +-y, +-z (two exactly zero
+    components, clamped reciprocals), along face diagonals (one zero component), and
+    origins snapped to the Cornell box's wall planes and to a coarse grid, so that rays
+    start on or graze box faces."""
+    rng = np.random.default_rng(seed)
+    dirs = []
+    for k in range(3):
+        for sgn in (1.0, -1.0):
+            d = np.zeros(3)
+            d[k] = sgn
+            dirs.append(d)
+    for k in range(3):
+        for s1 in (1.0, -1.0):
+            for s2 in (1.0, -1.0):
+                d = np.zeros(3)
+                d[(k + 1) % 3] = s1
+                d[(k + 2) % 3] = s2
+                dirs.append(d / np.linalg.norm(d))
+    dirs = np.array(dirs)
+    o = rng.uniform(0.0, 1.0, (n, 3))
+    snap = rng.integers(0, 4, (n, 3))
+    o = np.where(snap == 0, np.round(o * 8.0) / 8.0, o)    # coarse grid
+    o = np.where(snap == 1, np.round(o), o)                # on the 0 / 1 wall planes
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = 1e-4
+    r[:, 4:7] = dirs[rng.integers(0, len(dirs), n)]
+    r[:, 7] = 1e30
+    return r
+
+
+@pytest.mark.parametrize("scene", ["cornell", "soup"])
+def test_trace_axis_aligned_rays(rend, scene):
+    """Closest and any hits of axis-aligned and face-diagonal rays (zero direction
+    components) starting on wall planes and grid points equal the oracle's brute force."""
+    sc = scenes.cornell() if scene == "cornell" else scenes.soup(n_tris=50_000, seed=3, width=64, height=64)
+    rend.UpdateScene(sc)
+    orc = _oracle(sc)
+    rays = _axis_rays(20000, seed=21)
+    hg, ig = rend.trace_closest(rays)
+    hb, ib = orc.trace_closest(rays, brute=True)
+    _check_hits(hg, ig, hb, ib, f"{scene} axis rays gpu vs brute force")
+    sr = rays.copy()
+    sr[:, 7] = np.random.default_rng(6).uniform(0.01, 0.6, sr.shape[0])
+    assert np.array_equal(rend.trace_any(sr), orc.trace_any(sr))
+
+
 @pytest.mark.parametrize("max_leaf", [1, 7])
 def test_trace_soup_leaf_sizes(rend, max_leaf):
     """Leaves of 1 and of up to 7 triangles: with 7, a wave's leaf phase holds up to 896

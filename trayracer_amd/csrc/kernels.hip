@@ -145,26 +145,26 @@ struct TraceOut {
   uint32_t nodes, tests;
 };
 
-// Ray/box slab test of the device traversals: the host slab's arithmetic ((plane - o)
-// * inv, each distance within 2^-23 relative of exact, accepted with the relative
-// slack kBoxSlack, so no box the ray touches is culled and traversal equals a
-// brute-force loop), with reciprocals clamped to +-1e30 (zero and tiny components).
-// Every distance is then finite, so min/max can use the hardware v_min/v_max and
-// v_min3/v_max3 instead of compare-and-select pairs (the two agree on non-NaN inputs).
+// Ray/box slab test of the device traversals (Ize 2013's conservative form): each
+// distance is (plane - o) * inv, within 2^-23 relative of exact, and the far distance
+// is widened by kBoxSlack, so no box the ray touches is culled and traversal equals a
+// brute-force loop. inv = 1/d is the IEEE reciprocal: a zero (or denormal) component
+// gives +-inf, and a ray parallel to an axis then gets (-inf, +inf) inside the slab,
+// an empty interval outside it and, lying exactly on a bounding plane, 0 * inf = NaN.
+// Per axis, the entry and exit are the NaN-propagating minimum and maximum of the two
+// distances (v_minimum3/v_maximum3), and the axes are combined with max/min that ignore
+// NaN (v_max3/v_min3): an on-plane parallel axis limits neither end, as its true
+// unbounded interval would not. (Clamping the reciprocal to a large finite value instead
+// turns the on-plane case into a zero exit distance and culls the box.)
 struct RayBox {
   V3 o;
-  V3 inv;  // clamped reciprocal direction
+  V3 inv;  // 1 / d, infinite for a zero component
 };
-
-__device__ __forceinline__ float clamp_inv(float x) {
-  const float r = 1.f / x;
-  return __builtin_fabsf(r) <= 1e30f ? r : (x < 0.f ? -1e30f : 1e30f);
-}
 
 __device__ __forceinline__ RayBox ray_box_setup(V3 o, V3 d) {
   RayBox b;
   b.o = o;
-  b.inv = v3(clamp_inv(d.x), clamp_inv(d.y), clamp_inv(d.z));
+  b.inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
   return b;
 }
 
@@ -173,10 +173,11 @@ __device__ __forceinline__ bool slab_hw(float lox, float hix, float loy, float h
   const float tx0 = (lox - b.o.x) * b.inv.x, tx1 = (hix - b.o.x) * b.inv.x;
   const float ty0 = (loy - b.o.y) * b.inv.y, ty1 = (hiy - b.o.y) * b.inv.y;
   const float tz0 = (loz - b.o.z) * b.inv.z, tz1 = (hiz - b.o.z) * b.inv.z;
-  const float n = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tx0, tx1), __builtin_fminf(ty0, ty1)),
-                                  __builtin_fmaxf(__builtin_fminf(tz0, tz1), tmin));
-  const float f = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)),
-                                  __builtin_fminf(__builtin_fmaxf(tz0, tz1), tmax));
+  const float nx = __builtin_elementwise_minimum(tx0, tx1), fx = __builtin_elementwise_maximum(tx0, tx1);
+  const float ny = __builtin_elementwise_minimum(ty0, ty1), fy = __builtin_elementwise_maximum(ty0, ty1);
+  const float nz = __builtin_elementwise_minimum(tz0, tz1), fz = __builtin_elementwise_maximum(tz0, tz1);
+  const float n = __builtin_fmaxf(__builtin_fmaxf(nx, ny), __builtin_fmaxf(nz, tmin));
+  const float f = __builtin_fminf(__builtin_fminf(fx, fy), __builtin_fminf(fz, tmax));
   tnear = n;
   return n <= f * kBoxSlack;
 }

@@ -103,29 +103,13 @@ TRT_FI bool closer(float t, uint32_t key, float tbest, uint32_t keybest) {
   return t < tbest || (t == tbest && key < keybest);
 }
 
-// Conservative slab test (Ize 2013, "Robust BVH ray traversal"): each slab distance is
-// (bound - o) * inv with inv = 1/d, so its relative error is a few ulp whatever the
-// magnitude of o. The far distance is widened by kBoxSlack (~128 ulp), which covers
-// both the slab error and the error of a triangle's computed t, so a triangle lying
-// on a box face (axis-aligned geometry, equal-t ties) is never culled by rounding.
-// The traversal result therefore equals a brute-force loop over the same triangles.
+// Slack of the traversal's conservative slab test (Ize 2013, "Robust BVH ray traversal";
+// the test itself is in the device traversal): each slab distance is (bound - o) * inv
+// with inv = 1/d, so its relative error is a few ulp whatever the magnitude of o. The
+// far distance is widened by kBoxSlack (~128 ulp), which covers both the slab error and
+// the error of a triangle's computed t, so a triangle lying on a box face (axis-aligned
+// geometry, equal-t ties) is never culled by rounding. The traversal result therefore
+// equals a brute-force loop over the same triangles.
 constexpr float kBoxSlack = 1.0000153f;  // 1 + 2^-16
-TRT_FI bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz, V3 o, V3 inv, float tmin,
-                 float tmax, float& tnear) {
-  const float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
-  const float ty0 = (loy - o.y) * inv.y, ty1 = (hiy - o.y) * inv.y;
-  const float tz0 = (loz - o.z) * inv.z, tz1 = (hiz - o.z) * inv.z;
-  const float n = fmax(fmax(fmin(tx0, tx1), fmin(ty0, ty1)), fmax(fmin(tz0, tz1), tmin));
-  const float f = fmin(fmin(fmax(tx0, tx1), fmax(ty0, ty1)), fmin(fmax(tz0, tz1), tmax));
-  tnear = n;
-  return n <= f * kBoxSlack;
-}
-
-// reciprocal direction; a zero component becomes a large finite value of the same
-// sign convention (+), which keeps every slab distance finite or +-inf, never NaN
-TRT_FI V3 safe_inv(V3 d) {
-  auto r = [](float x) { return x == 0.f ? 1e30f : 1.f / x; };
-  return v3(r(d.x), r(d.y), r(d.z));
-}
 
 }  // namespace trt
