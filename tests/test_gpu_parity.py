@@ -74,8 +74,8 @@ def test_trace_soup_exact(rend, presplit):
 
 
 def _axis_rays(n, seed):
-    """Degenerate rays for the slab test: directions along +-x, +-y, +-z (two exactly zero
-    components, clamped reciprocals), along face diagonals (one zero component), and
+    """Degenerate rays for the slab test: directions along +-x, +-y, +-z (two zero
+    components, +0.0 or -0.0), along face diagonals (one zero component), and
     origins snapped to the Cornell box's wall planes and to a coarse grid, so that rays
     start on or graze box faces."""
     rng = np.random.default_rng(seed)
@@ -100,7 +100,9 @@ def _axis_rays(n, seed):
     r = np.zeros((n, 8), np.float32)
     r[:, 0:3] = o
     r[:, 3] = 1e-4
-    r[:, 4:7] = dirs[rng.integers(0, len(dirs), n)]
+    d = dirs[rng.integers(0, len(dirs), n)]
+    neg0 = (d == 0.0) & (rng.random(d.shape) < 0.5)  # -0.0 components: reciprocal -inf
+    r[:, 4:7] = np.where(neg0, -0.0, d)
     r[:, 7] = 1e30
     return r
 
