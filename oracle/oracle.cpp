@@ -173,7 +173,7 @@ inline bool test_tri(const OScene& s, uint32_t off, uint32_t local, const RayTri
     return true;
   }
   if (th.t < b.t || inst < b.inst || (inst == b.inst && local < b.prim)) {
-    b.t = th.t;
+    b.t = th.t + 0.f;  // a hit at t = 0 is recorded as +0: the sign of zero carries no meaning
     b.b1 = th.b1;
     b.b2 = th.b2;
     b.prim = local;

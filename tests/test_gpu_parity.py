@@ -73,7 +73,7 @@ def test_trace_soup_exact(rend, presplit):
     assert np.array_equal(rend.trace_any(sr), orc.trace_any(sr))
 
 
-def _axis_rays(n, seed):
+def _axis_rays(n, seed, scale=1.0):
     """Degenerate rays for the slab test: directions along +-x, +-y, +-z (two zero
     components, +0.0 or -0.0), along face diagonals (one zero component), and
     origins snapped to the Cornell box's wall planes and to a coarse grid, so that rays
@@ -98,8 +98,8 @@ def _axis_rays(n, seed):
     o = np.where(snap == 0, np.round(o * 8.0) / 8.0, o)    # coarse grid
     o = np.where(snap == 1, np.round(o), o)                # on the 0 / 1 wall planes
     r = np.zeros((n, 8), np.float32)
-    r[:, 0:3] = o
-    r[:, 3] = 1e-4
+    r[:, 0:3] = o * scale
+    r[:, 3] = np.where(rng.random(n) < 0.5, 0.0, 1e-4)  # tmin 0: hits at an origin on a wall
     d = dirs[rng.integers(0, len(dirs), n)]
     neg0 = (d == 0.0) & (rng.random(d.shape) < 0.5)  # -0.0 components: reciprocal -inf
     r[:, 4:7] = np.where(neg0, -0.0, d)
@@ -192,6 +192,10 @@ def test_instanced_exact(rend):
     hr, ir = orc.trace_closest(rays)
     _check_hits(hg, ig, hr, ir, "instanced gpu vs oracle")
     assert (prim_of(hg) != 0xFFFFFFFF).any()
+    ar = _axis_rays(20000, seed=22, scale=100.0)  # axis-aligned rays through the instance field
+    hg, ig = rend.trace_closest(ar)
+    hr, ir = orc.trace_closest(ar)
+    _check_hits(hg, ig, hr, ir, "instanced axis rays gpu vs oracle")
     rend.RenderFrame()
     ref = orc.render()
     _compare_paths(rend.path_radiance(), ref["path_rgb"], "instanced")
