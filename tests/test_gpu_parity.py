@@ -123,6 +123,26 @@ def test_trace_axis_aligned_rays(rend, scene):
     assert np.array_equal(rend.trace_any(sr), orc.trace_any(sr))
 
 
+@pytest.mark.parametrize("n_tris", [1, 2, 3])
+def test_trace_tiny_scenes(rend, n_tris):
+    """Scenes of 1-3 triangles: the tree's root is a leaf, stored as a node with an empty
+    second child (a zero box at the origin, which rays through the origin visit)."""
+    import copy
+    sc = copy.deepcopy(scenes.cornell())
+    sc["positions"] = np.asarray(sc["positions"], np.float32)[:n_tris]
+    sc["mat_ids"] = np.asarray(sc["mat_ids"])[:n_tris]
+    sc["meshes"] = [(0, n_tris)]
+    rend.UpdateScene(sc)
+    orc = _oracle(sc)
+    rays = np.concatenate([random_rays(4000, seed=30 + n_tris, lo=-0.2, hi=1.2, tmin=1e-4, toward=(0.0, 1.0)),
+                           _axis_rays(4000, seed=40 + n_tris)])
+    rays[:500, 0:3] = 0.0  # from the origin, where the empty child's box sits
+    hg, ig = rend.trace_closest(rays)
+    hb, ib = orc.trace_closest(rays, brute=True)
+    _check_hits(hg, ig, hb, ib, f"{n_tris}-triangle scene gpu vs brute force")
+    assert np.array_equal(rend.trace_any(rays), orc.trace_any(rays, brute=True))
+
+
 @pytest.mark.parametrize("max_leaf", [1, 7])
 def test_trace_soup_leaf_sizes(rend, max_leaf):
     """Leaves of 1 and of up to 7 triangles: with 7, a wave's leaf phase holds up to 896
