@@ -221,6 +221,28 @@ def test_instanced_exact(rend):
     _compare_paths(rend.path_radiance(), ref["path_rgb"], "instanced")
 
 
+def test_instanced_duplicates_tie_to_lower_instance(rend):
+    """Instances with identical transforms give identical hits; the (t, instance, prim)
+    order picks the lower instance id, whatever the traversal visits first."""
+    sc = scenes.instanced(n_instances=8, subdiv=2, seed=5, width=16, height=16, spp=1, max_depth=2,
+                          emissive_per_instance=1)
+    inst = np.asarray(sc["instances"])
+    sc["instances"] = np.concatenate([inst, inst[[3, 0, 5, 0]]])  # ids 8-11 repeat 3, 0, 5, 0
+    rend.UpdateScene(sc)
+    orc = _oracle(sc)
+    rays = random_rays(20000, seed=17, lo=-10, hi=110, toward=(0, 100), tmin=1e-4)
+    # aim half of them at the repeated instances' translations
+    tgt = inst[[3, 0, 5], :][:, [3, 7, 11]]
+    aim = tgt[np.arange(10000) % 3] + np.random.default_rng(3).normal(0.0, 0.5, (10000, 3))
+    d = aim - rays[:10000, 0:3]
+    rays[:10000, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    hg, ig = rend.trace_closest(rays)
+    hb, ib = orc.trace_closest(rays, brute=True)
+    _check_hits(hg, ig, hb, ib, "duplicated instances gpu vs brute force")
+    hit = prim_of(hb) != 0xFFFFFFFF
+    assert hit.sum() > 1000 and not np.isin(ib[hit], [8, 9, 10, 11]).any()
+
+
 def test_progressive_accumulation(rend):
     sc = scenes.cornell(width=32, height=32, spp=2, seed=1)
     rend.UpdateScene(sc)
