@@ -391,11 +391,12 @@ __device__ __forceinline__ bool traverse_ww(const DevNode48* __restrict__ nodes,
 // finish(slot, r, hit) consumes a result (hit: a hit was found).
 constexpr uint32_t kRefill = 40;
 
-// Traversal stack of the single-level kernels: the top kRingEntries entries live in an
+// Traversal stack of the queue traversals: the top kRingEntries entries live in an
 // LDS ring (entry k in slot k % kRingEntries, lane-interleaved), older ones are moved
 // to the lane's area of a global spill buffer when the ring is full and brought back,
 // a ring at a time, when it runs empty. Deep traversals are rare, so the ring keeps
-// LDS small (4 KiB per wave) and occupancy bound by registers, not by the tree depth.
+// LDS small (4 KiB per wave, beside the 1.6 KiB leaf-task area) and occupancy bound by
+// registers, not by the tree depth.
 constexpr int kRingEntries = 16;
 
 struct TwoTierStack {
