@@ -143,6 +143,31 @@ def test_trace_tiny_scenes(rend, n_tris):
     assert np.array_equal(rend.trace_any(rays), orc.trace_any(rays, brute=True))
 
 
+def test_empty_scene_is_rejected():
+    """A scene without triangles is refused by the build with an error (host check, no
+    kernel runs); the context stays usable for a valid scene afterwards, and a trace of
+    no rays returns empty results."""
+    import copy
+
+    from trayracer_amd import Renderer
+    from trayracer_amd._lib import TrtError
+    r = Renderer(0)
+    sc = copy.deepcopy(scenes.cornell())
+    sc["positions"] = np.zeros((0, 3, 3), np.float32)
+    sc["mat_ids"] = np.zeros(0, np.uint32)
+    sc["meshes"] = []
+    with pytest.raises(TrtError, match="no triangles"):
+        r.UpdateScene(sc)
+    good = scenes.cornell()
+    r.UpdateScene(good)
+    rays = random_rays(1000, seed=5, lo=-0.2, hi=1.2, tmin=1e-4, toward=(0.0, 1.0))
+    hg, ig = r.trace_closest(rays)
+    hb, ib = _oracle(good).trace_closest(rays, brute=True)
+    _check_hits(hg, ig, hb, ib, "cornell after a refused empty scene")
+    h0, i0 = r.trace_closest(rays[:0])  # no rays: nothing launched, empty results
+    assert len(h0) == 0 and len(i0) == 0 and len(r.trace_any(rays[:0])) == 0
+
+
 @pytest.mark.parametrize("max_leaf", [1, 7])
 def test_trace_soup_leaf_sizes(rend, max_leaf):
     """Leaves of 1 and of up to 7 triangles: with 7, a wave's leaf phase holds up to 896

@@ -598,6 +598,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   const uint32_t n_tris = uint32_t(ctx->mat_ids.size());
   const uint32_t n_mats = uint32_t(ctx->materials.size());
   if (n_mats == 0) return fail(ctx, TRT_INVALID_ARG, "no materials");
+  if (n_tris == 0) return fail(ctx, TRT_INVALID_ARG, "scene has no triangles");
   for (uint32_t g = 0; g < n_tris; ++g)
     if (ctx->mat_ids[g] >= n_mats && ctx->instances_in.empty())
       return fail(ctx, TRT_INVALID_ARG, "material id out of range at triangle " + std::to_string(g));
