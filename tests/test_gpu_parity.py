@@ -168,6 +168,27 @@ def test_empty_scene_is_rejected():
     assert len(h0) == 0 and len(i0) == 0 and len(r.trace_any(rays[:0])) == 0
 
 
+def test_non_finite_rays_are_rejected(rend):
+    """Rays with a NaN or infinite origin or direction component, or a NaN bound, are
+    refused before anything is launched; an infinite tmax is a valid unbounded ray."""
+    from trayracer_amd._lib import TrtError
+    sc = scenes.cornell()
+    rend.UpdateScene(sc)
+    rays = random_rays(256, seed=6, lo=-0.2, hi=1.2, tmin=1e-4, toward=(0.0, 1.0))
+    for col, val in ((0, np.nan), (2, np.inf), (5, -np.inf), (3, np.nan), (7, np.nan)):
+        bad = rays.copy()
+        bad[100, col] = val
+        with pytest.raises(TrtError, match="ray 100 "):
+            rend.trace_closest(bad)
+        with pytest.raises(TrtError, match="ray 100 "):
+            rend.trace_any(bad)
+    unbounded = rays.copy()
+    unbounded[:, 7] = np.inf
+    hg, ig = rend.trace_closest(unbounded)
+    hb, ib = _oracle(sc).trace_closest(unbounded, brute=True)
+    _check_hits(hg, ig, hb, ib, "cornell with tmax = +inf")
+
+
 @pytest.mark.parametrize("max_leaf", [1, 7])
 def test_trace_soup_leaf_sizes(rend, max_leaf):
     """Leaves of 1 and of up to 7 triangles: with 7, a wave's leaf phase holds up to 896

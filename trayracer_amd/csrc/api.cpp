@@ -889,6 +889,10 @@ static trt_status trace_common(trt_context* ctx, const float* rays, uint32_t n, 
   // (o, tmin) rows first, then (d, tmax) rows
   std::vector<float> soa(size_t(n) * 8);
   for (uint32_t i = 0; i < n; ++i) {
+    const float* r = &rays[size_t(i) * 8];
+    // origins and directions finite, interval bounds not NaN (tmax may be +inf)
+    if (!finite_all(r, 3) || !finite_all(r + 4, 3) || r[3] != r[3] || r[7] != r[7])
+      return fail(ctx, TRT_INVALID_ARG, "ray " + std::to_string(i) + " has a non-finite origin or direction or a NaN bound");
     std::memcpy(&soa[size_t(i) * 4], &rays[size_t(i) * 8], 16);
     std::memcpy(&soa[(size_t(n) + i) * 4], &rays[size_t(i) * 8 + 4], 16);
   }
