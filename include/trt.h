@@ -186,7 +186,8 @@ trt_status trt_add_mesh(trt_context* ctx, const float* positions, const float* n
 /* Two-level scenes: when instances are set, every mesh is traced through them. */
 trt_status trt_set_instances(trt_context* ctx, const trt_instance* inst, uint32_t n);
 /* Builds the BVH(s), the light table, and uploads everything (replaces Model::Upload +
- * OptixRenderer::BuildAccel + Scene::GatherLights, Tracer/Renderer/Scene.cpp:275-302). */
+ * OptixRenderer::BuildAccel + Scene::GatherLights, Tracer/Renderer/Scene.cpp:275-302).
+ * TRT_INVALID_ARG when the scene has no triangles or no materials. */
 trt_status trt_build(trt_context* ctx, const trt_build_opts* opts);
 trt_status trt_bvh_info_get(trt_context* ctx, trt_bvh_info* info);
 
@@ -295,7 +296,9 @@ trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths
  * Tracer/Optix/GPU/optix.cu:264-265, 333-335). rays: 8 floats per ray
  * (ox, oy, oz, tmin, dx, dy, dz, tmax). hits: 4 floats per ray (t, b1, b2, prim as uint
  * bits; prim = 0xffffffff on a miss); inst: instance per ray (0xffffffff single-level).
- * Both run the renderer's own Extend / Shadow traversal kernels over the rays. */
+ * Both run the renderer's own Extend / Shadow traversal kernels over the rays.
+ * TRT_INVALID_ARG, before anything is launched, for a ray with a non-finite origin or
+ * direction component or a NaN tmin/tmax (tmax = +inf is allowed); n = 0 does nothing. */
 trt_status trt_trace_closest(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst);
 trt_status trt_trace_any(trt_context* ctx, const float* rays, uint32_t n, uint8_t* occluded);
 
