@@ -166,6 +166,7 @@ def test_empty_scene_is_rejected():
     _check_hits(hg, ig, hb, ib, "cornell after a refused empty scene")
     h0, i0 = r.trace_closest(rays[:0])  # no rays: nothing launched, empty results
     assert len(h0) == 0 and len(i0) == 0 and len(r.trace_any(rays[:0])) == 0
+    r.close()
 
 
 def test_non_finite_rays_are_rejected(rend):
@@ -236,6 +237,19 @@ def test_path_radiance_cornell_rr_deep(rend):
     rend.RenderFrame()
     ref = _oracle(sc).render()
     _compare_paths(rend.path_radiance(), ref["path_rgb"], "cornell rr")
+
+
+@pytest.mark.parametrize("width,height,spp", [(1, 1, 1), (3, 5, 2), (67, 1, 3), (1, 130, 1)])
+def test_path_radiance_ragged_frames(rend, width, height, spp):
+    """Frames whose path counts are far from a whole wave or queue segment (1 path up to
+    201), including single-row and single-column films."""
+    sc = scenes.cornell(width=width, height=height, spp=spp, max_depth=4, rr_depth=3, seed=13)
+    rend.UpdateScene(sc)
+    rend.RenderFrame()
+    ref = _oracle(sc).render()
+    assert rend.path_radiance().shape[0] == width * height * spp
+    _compare_paths(rend.path_radiance(), ref["path_rgb"], f"cornell {width}x{height}x{spp}")
+    np.testing.assert_allclose(rend.color()[..., :3], ref["image_f32"], rtol=1e-5, atol=1e-7)
 
 
 @pytest.mark.parametrize("presplit", [0.0, 1.0, 6.0])
