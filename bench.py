@@ -240,6 +240,10 @@ def main():
         "trace_mrays_s": round((ext_rays + sh_rays) / (trace_ms * 1e-3) / 1e6, 1),
         "extend_mrays_s": round(ext_rays / (ext_ms * 1e-3) / 1e6, 1),
         "shadow_mrays_s": round(sh_rays / (sh_ms * 1e-3) / 1e6, 1) if sh_ms > 0 else None,
+        "rates_note": "trace_mrays_s: Extend + Shadow rays over the wall time of both phases; extend_mrays_s and "
+                      "shadow_mrays_s: each phase's rays over its own launch intervals, and every Shadow launch "
+                      "shares the GPU with the next bounce's Extend (second stream), so these understate the "
+                      "kernels' own rates",
         "gpu_frame_ms": round(frame_ms / args.steps, 3),
         "phase_ms_per_frame": {"extend": round(ext_ms / args.steps, 3), "shade": round(shade_ms / args.steps, 3),
                                "shadow": round(sh_ms / args.steps, 3), "trace_wall": round(trace_ms / args.steps, 3),
