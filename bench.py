@@ -250,7 +250,8 @@ def main():
                                "note": "extend and shadow are per-launch intervals that overlap across bounces; "
                                        "trace_wall is the wall time of both phases"},
         "bvh": {"nodes": info["nodes"], "depth": info["depth"], "sah_cost": round(info["sah_cost"], 2),
-                "presplit": args.presplit, "references": info["references"], "build_s": round(build_s, 2)},
+                "presplit": args.presplit, "references": info["references"], "build_s": round(build_s, 2),
+                "trt_build_s": round(info["build_ms"] / 1e3, 2)},
         "roofline": roofline, "cpu_baseline": cpu, "time_to_relmse": dn,
     }
     print(json.dumps(line), flush=True)

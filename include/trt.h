@@ -75,7 +75,7 @@ typedef struct trt_camera {
 typedef struct trt_build_opts {
   int32_t bins;      /* binned-SAH bins per axis (32) */
   int32_t max_leaf;  /* primitives per leaf, 1..7 (4) */
-  int32_t threads;   /* host build threads (0 = hardware concurrency) */
+  int32_t threads;   /* host build threads (0 = OMP_NUM_THREADS if set, else the CPUs of the process affinity mask) */
   float presplit;    /* spatial pre-splitting budget: extra BVH references per triangle
                         (0 = off; 1.0 roughly quarters the triangle tests on scenes of
                         large overlapping triangles). Results do not depend on it. */

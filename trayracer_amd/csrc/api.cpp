@@ -8,9 +8,11 @@
 // device; the reference downloads counters every bounce, DeviceRenderer.cpp:256), and
 // errors are returned as trt_status instead of exit() (Tracer/Utility/Errors.cpp:15-37).
 #include <hip/hip_runtime.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <random>
 #include <chrono>
@@ -586,6 +588,21 @@ trt_status trt_set_instances(trt_context* ctx, const trt_instance* inst, uint32_
   return TRT_OK;
 }
 
+// Host threads for the BVH build when the caller gives none: OMP_NUM_THREADS when set
+// (the usual per-process thread budget on shared machines, where the hardware thread
+// count can be far above the CPUs this process may use), else the CPUs in this
+// process's affinity mask, else the hardware thread count.
+static int default_build_threads() {
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) return v;
+  }
+  cpu_set_t set;
+  CPU_ZERO(&set);
+  if (sched_getaffinity(0, sizeof(set), &set) == 0 && CPU_COUNT(&set) > 0) return CPU_COUNT(&set);
+  return int(std::thread::hardware_concurrency());
+}
+
 trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   if (!ctx) return TRT_INVALID_ARG;
   trt_build_opts o = opts ? *opts : trt_build_opts{32, 4, 0, 0.f};
@@ -593,7 +610,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
     return fail(ctx, TRT_INVALID_ARG, "bins must be in [2,256] and max_leaf in [1,7]");
   if (!(o.presplit >= 0.f && o.presplit <= 8.f))
     return fail(ctx, TRT_INVALID_ARG, "presplit must be in [0, 8]");
-  int threads = o.threads > 0 ? o.threads : int(std::thread::hardware_concurrency());
+  int threads = o.threads > 0 ? o.threads : default_build_threads();
   if (threads <= 0) threads = 1;
   const uint32_t n_tris = uint32_t(ctx->mat_ids.size());
   const uint32_t n_mats = uint32_t(ctx->materials.size());

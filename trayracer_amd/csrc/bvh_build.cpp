@@ -390,8 +390,28 @@ BuildResult build_binned_sah(const float* prim_box_data, uint32_t n, int bins, i
 namespace {
 
 // The float nearest to v in the outward direction (down for a lower bound, up for an
-// upper one) whose low byte is `byte`; false if none is finite.
+// upper one) whose low byte is `byte`; false if none is finite. Stepping outward moves
+// the bit pattern by one per step (up for a positive value going up or a negative one
+// going down, where the magnitude grows; down otherwise), so the answer is found
+// directly unless the steps would cross zero, which the step-by-step walk handles.
 bool carry_byte(float v, uint32_t byte, bool upper, float& out) {
+  if (std::isfinite(v)) {
+    const uint32_t u = f2u(v), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
+    const bool grows = (sign == 0u) == upper;  // magnitude grows with each step
+    if (grows) {
+      const uint32_t m = mag + ((byte - mag) & 0xffu);
+      if (m < 0x7f800000u) {
+        out = u2f(sign | m);
+        return true;
+      }
+      return false;  // the walk reaches infinity first
+    }
+    const uint32_t k = (mag - byte) & 0xffu;
+    if (k <= mag) {  // k = mag lands on a zero of v's sign, as the walk does
+      out = u2f(sign | (mag - k));
+      return true;
+    }
+  }
   float x = v;
   for (int k = 0; k <= 512; ++k) {
     if ((f2u(x) & 0xffu) == byte && std::isfinite(x)) {
