@@ -58,6 +58,12 @@ int main(int argc, char** argv) {
       for (int r = 0; r < 2; ++r) {
         if (upper ? !(s[r] >= e[r]) : !(s[r] <= e[r])) return fail("bound moved inward", i);
         if (ulp_dist(s[r], e[r]) > 256) return fail("bound moved more than 256 ulp", i);
+        // nearest: stepping outward one ulp at a time, the first float with the carried
+        // low byte is the packed bound
+        float x = e[r];
+        const uint32_t want = trt::f2u(s[r]) & 0xffu;
+        while ((trt::f2u(x) & 0xffu) != want) x = std::nextafter(x, upper ? HUGE_VALF : -HUGE_VALF);
+        if (trt::f2u(x) != trt::f2u(s[r])) return fail("packed bound is not the nearest carrying its byte", i);
       }
       if (std::memcmp(&p.r2[k], &a.r2[k], 4) != 0) return fail("z bound changed", i);
     }
