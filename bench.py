@@ -6,9 +6,10 @@ Finalize, all on one MI355X with the scene already resident in HBM. `value` is t
 whole-frame throughput: rays traced (extension + shadow) / wall time of the timed
 frames. Traversal-only throughput (rays / wall time of the Extend and Shadow phases from
 HIP events; Shadow of bounce d runs beside Extend of bounce d + 1 on a second stream) is
-reported beside it, with the roofline of the Extend kernel computed from algorithmic
-bytes (ray + hit records, node visits x node bytes, triangle tests x triangle bytes;
-visit counts from a counting pass over the same frame).
+reported beside it, with two rooflines of the Extend kernel (HBM: the bytes a launch
+must move at least, against 8 TB/s; vector memory: the bytes every ray loads, against the
+measured rate of random 48-byte gathers from the L2; counts from a counting pass over the
+same frame) and Shade's HBM roofline.
 """
 import argparse
 import json
@@ -82,25 +83,66 @@ def make_scene(name):
     return scenes.instanced(), "instanced-1000x20480-tris 3840x2160 4spp depth6"
 
 
-def cpu_baseline(scene, threads, spp=2):
-    """Oracle (host build of the same per-vertex math) on a bounded sample of the same
-    workload: the same camera and resolution at `spp` samples per pixel (8 in the GPU
-    frame), same depth, about 10 s of host work on 16 threads."""
+def host_cpu():
+    """CPU model and the host threads this process may use: the CPUs of its affinity mask,
+    capped by OMP_NUM_THREADS when that is set (the per-process CPU share of a shared
+    machine)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return model, affinity, (min(affinity, omp) if omp > 0 else affinity)
+
+
+CPU_CROP = (832, 412, 256, 256)  # the 1080p film's centre
+
+
+def cpu_baseline(threads, presplit, crop_spp=128):
+    """The host build of the same per-vertex code (camera, traversal, BSDF, integrator)
+    on the host cores, traversing the product's own binned-SAH tree over pre-split
+    references (the GPU's tree, built by the same host code), near child first: the
+    Cornell configuration (256x256, 16 spp, depth 4) in full, and the 1M-triangle soup's
+    frame on a fixed 256x256 window at the film's centre, the same paths as the GPU
+    frame's. One spp of that window is only ~0.2 M rays, so `crop_spp` samples are
+    traced for a stable rate."""
     from oracle import Oracle
+    from trayracer_amd import scenes
+    model, affinity, _ = host_cpu()
+    sc0 = scenes.cornell()
+    orc = Oracle(sc0)
+    orc.use_product_bvh(presplit, threads)
     t0 = time.perf_counter()
-    orc = Oracle(scene)
-    build_s = time.perf_counter() - t0
-    r = dict(scene["render"])
-    W, H = r["width"], r["height"]
-    t0 = time.perf_counter()
-    out = orc.render(render=dict(width=W, height=H, spp=spp), threads=threads)
-    dt = time.perf_counter() - t0
-    rays = int(out["counts"].sum())
+    out0 = orc.render(threads=threads)
+    dt0 = time.perf_counter() - t0
+    rays0 = int(out0["counts"].sum())
     orc.close()
+    sc1 = scenes.soup()
+    t0 = time.perf_counter()
+    orc = Oracle(sc1)
+    orc.use_product_bvh(presplit, threads)
+    build_s = time.perf_counter() - t0
+    x0, y0, w, h = CPU_CROP
+    t0 = time.perf_counter()
+    out1 = orc.render(render=dict(spp=crop_spp), crop=CPU_CROP, threads=threads)
+    dt1 = time.perf_counter() - t0
+    rays1 = int(out1["counts"].sum())
+    orc.close()
+    r = sc1["render"]
     return {
-        "value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-        "sample": f"same scene and camera at {W}x{H}, {spp} spp, depth {r['max_depth']}: {rays} rays in {dt:.2f} s "
-                  f"(oracle BVH build {build_s:.1f} s not timed)",
+        "value": round(rays1 / dt1 / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+        "sample": f"1M-triangle soup frame ({r['width']}x{r['height']}, depth {r['max_depth']}), window {w}x{h} at "
+                  f"({x0}, {y0}), {crop_spp} spp: {rays1} rays in {dt1:.2f} s on {threads} threads; the product's "
+                  f"pre-split SAH tree (build {build_s:.1f} s, not timed)",
+        "cpu_model": model, "affinity_cpus": affinity,
+        "cornell_full": {"config": "256x256, 16 spp, depth 4, RR from depth 3", "seconds": round(dt0, 3),
+                         "rays": rays0, "mrays_s": round(rays0 / dt0 / 1e6, 3)},
     }
 
 
@@ -162,6 +204,106 @@ def denoise_metric(rend, steps):
             "instanced_view": pick(inst), "network_roofline": network_roofline(rend)}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_extend_sl_soup1m_traffic.json")
+# the sources whose code the measured traffic belongs to
+TRAVERSAL_SOURCES = ["trayracer_amd/csrc/kernels.hip", "trayracer_amd/csrc/device.h", "trayracer_amd/csrc/bvh_build.cpp",
+                     "trayracer_amd/csrc/bvh_build.h", "trayracer_amd/csrc/api.cpp", "trayracer_amd/csrc/trt/intersect.h"]
+
+
+def traversal_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for rel in TRAVERSAL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def gather_ceilings():
+    """Rates of independent 48-byte random gathers (trt_measure_gather) from a buffer that
+    fits the L2 (4 MiB), one that fits the Infinity Cache (128 MiB) and one in HBM (4 GiB)."""
+    import ctypes
+
+    from trayracer_amd import _lib
+    lib = _lib.load()
+    out = {}
+    for name, size in (("l2_4MiB", 4 << 20), ("mall_128MiB", 128 << 20), ("hbm_4GiB", 4 << 30)):
+        g = ctypes.c_double()
+        st = lib.trt_measure_gather(0, size, 256, ctypes.byref(g))
+        out[name] = round(g.value, 1) if st == 0 else None
+    return out
+
+
+def extend_roofline(tc, info, ext_rays, ext_ms, launches, workload):
+    """Two ceilings for the closest-hit traversal (Extend), per launch, from HIP-event
+    launch times:
+    * HBM: the bytes a launch must move at least, its rays' records (32 B ray in, 16 B hit
+      out) plus every distinct node and triangle record it reads once (48 B each; counted
+      by the counting pass's touch bitmaps), against the 8 TB/s peak; `traffic` is the
+      rocprofv3 measurement of the same kernel (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+      null when the profile was taken from other kernel sources;
+    * vector memory: the bytes every ray loads (ray and hit records, node visits x 48 B,
+      triangle tests x 48 B) against the rate of independent random 48-byte gathers from
+      an L2-resident buffer (trt_measure_gather), the fastest such loads go on this GPU."""
+    n = max(1, launches)
+    avg_s = ext_ms / n * 1e-3
+    per_ray = 48 + tc["closest_nodes_per_ray"] * tc["node_bytes"] + tc["closest_tris_per_ray"] * tc["tri_bytes"]
+    distinct = (tc["closest_distinct_nodes"] * tc["node_bytes"] + tc["closest_distinct_tris"] * tc["tri_bytes"]) / \
+        max(1, tc["launches"])
+    rays_per_launch = ext_rays / n
+    compulsory = 48 * rays_per_launch + distinct
+    achieved = compulsory / avg_s / 1e9
+    traffic, stale = None, None
+    if workload == "soup1m" and os.path.exists(TRAFFIC_PROFILE):
+        with open(TRAFFIC_PROFILE) as f:
+            prof = json.load(f)
+        stale = prof.get("kernel_source_sha16") != traversal_source_hash()
+        if not stale:
+            traffic = round(prof["traffic_bytes_per_launch"])
+    ceil = gather_ceilings()
+    loaded = per_ray * rays_per_launch / avg_s / 1e9
+    vm_peak = ceil["l2_4MiB"]
+    return {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated "
+                          "(profiles/k_extend_sl_soup1m_hbm_traffic.md)" if traffic else None,
+        "traffic_stale": stale,
+        "dram_gbs": round(traffic / avg_s / 1e9, 1) if traffic else None,
+        "dram_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+        "kernel": "k_extend_tl<false>" if info["two_level"] else "k_extend_sl<false>",
+        "avg_launch_ms": round(ext_ms / n, 4), "rays_per_launch": round(rays_per_launch),
+        "algorithmic_bytes_per_launch": round(compulsory),
+        "algorithmic_bytes": "48 B per ray (ray in, hit out) + 48 B per distinct node and triangle record read",
+        "vector_memory": {"achieved": round(loaded, 1), "peak": vm_peak, "unit": "GB/s",
+                          "frac": round(loaded / vm_peak, 4) if vm_peak else None,
+                          "bytes_per_ray": round(per_ray, 1), "nodes_per_ray": round(tc["closest_nodes_per_ray"], 2),
+                          "tris_per_ray": round(tc["closest_tris_per_ray"], 2),
+                          "peak_source": "trt_measure_gather: independent random 48-byte gathers, 4 MiB buffer",
+                          "gather_ceilings_gbs": ceil},
+        "stack_spills_per_ray": round(tc["closest_spills"] / max(1, tc["closest_rays"]), 4),
+    }
+
+
+def shade_roofline(scene, stats, steps, info):
+    """Shade's HBM roofline: algorithmic bytes per shaded vertex (the surface's records and
+    the path's queue traffic) over the frame's Shade time. Per live path: queue record
+    in (52 B: origin, direction, throughput, path) + hit (16 B, + 4 B instance) + triangle
+    positions (36 B, + 36 B normals) + material id (4 B) + material (80 B) + light record
+    and alias entry (88 B) + radiance read-modify-write (32 B) + extension record out (52 B)
+    + shadow record out (48 B)."""
+    two = bool(info["two_level"])
+    nrm = scene.get("normals") is not None
+    per = 52 + 16 + (4 if two else 0) + 36 + (36 if nrm else 0) + 4 + 80 + 88 + 32 + 52 + 48
+    ext = sum(s["extend_rays"] for s in stats) / steps
+    ms = sum(s["shade_ms"] for s in stats) / steps
+    achieved = per * ext / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_path": per, "paths_per_frame": round(ext),
+            "shade_ms_per_frame": round(ms, 3),
+            "valu_profile": "profiles/wavefront_soup1m_counters.md (VALU load, lane use, TA/TD busy per kernel)"}
+
+
 def main():
     args = parse()
     if args.copy_calibrate:
@@ -200,33 +342,15 @@ def main():
     launches = sum(s["extend_launches"] for s in stats)
 
     tc = rend.measure_traversal()
-    ray_io = 32 + 16  # ray record in, hit record out
-    bytes_per_ext = ray_io + tc["closest_nodes_per_ray"] * tc["node_bytes"] + tc["closest_tris_per_ray"] * tc["tri_bytes"]
-    ext_bytes = bytes_per_ext * ext_rays
-    achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
-    traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", "k_extend_sl_soup1m_traffic.json")
-    if args.workload == "soup1m" and os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = round(json.load(f)["traffic_bytes_per_launch"])
-        traffic_src = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/k_extend_sl_soup1m_hbm_traffic.md)"
-    avg_launch_s = ext_ms / max(1, launches) * 1e-3
-    roofline = {
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-        "kernel": "k_extend" if info["two_level"] else "k_extend_sl<false>",
-        "avg_launch_ms": round(ext_ms / max(1, launches), 4),
-        "dram_gbs": round(traffic / avg_launch_s / 1e9, 1) if traffic else None,
-        "bytes_per_ray": round(bytes_per_ext, 1), "nodes_per_ray": round(tc["closest_nodes_per_ray"], 2),
-        "tris_per_ray": round(tc["closest_tris_per_ray"], 2),
-    }
+    roofline = extend_roofline(tc, info, ext_rays, ext_ms, launches, args.workload)
+    shade = shade_roofline(scene, stats, args.steps, info)
     dn = None
     if not args.no_denoise:
         dn = denoise_metric(rend, args.denoise_steps)
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(scene, threads)
+        threads = args.cpu_threads or host_cpu()[2]
+        cpu = cpu_baseline(threads, args.presplit)
 
     line = {
         "metric": "Mrays/s (1080p, 1M tris); time to rel-MSE 0.01: plain vs denoised",
@@ -252,7 +376,7 @@ def main():
         "bvh": {"nodes": info["nodes"], "depth": info["depth"], "sah_cost": round(info["sah_cost"], 2),
                 "presplit": args.presplit, "references": info["references"], "build_s": round(build_s, 2),
                 "trt_build_s": round(info["build_ms"] / 1e3, 2)},
-        "roofline": roofline, "cpu_baseline": cpu, "time_to_relmse": dn,
+        "roofline": roofline, "shade_roofline": shade, "cpu_baseline": cpu, "time_to_relmse": dn,
     }
     print(json.dumps(line), flush=True)
     rend.close()

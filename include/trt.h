@@ -280,6 +280,10 @@ trt_status trt_set_camera(trt_context* ctx, const trt_camera* cam);
 trt_status trt_render(trt_context* ctx, const trt_render_opts* opts, trt_stats* stats);
 trt_status trt_reset(trt_context* ctx);
 uint32_t trt_sample_count(const trt_context* ctx);
+/* Number of frames this context has rendered (trt_render and trt_measure_traversal each
+ * count one); lets a caller tell whether the frame it rendered is still the last one
+ * (the reference keeps the equivalent per-frame counters in RenderStatistics). */
+uint64_t trt_frame_id(const trt_context* ctx);
 
 /* Outputs (replace the accumulator / colour / albedo / normal buffers,
  * Tracer/Renderer/DeviceRenderer.h:92-95, FinalizeFrame, Tracer/CUDA/GPU/CudaUtilityKernels.h:27-44).
@@ -298,7 +302,8 @@ trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths
  * bits; prim = 0xffffffff on a miss); inst: instance per ray (0xffffffff single-level).
  * Both run the renderer's own Extend / Shadow traversal kernels over the rays.
  * TRT_INVALID_ARG, before anything is launched, for a ray with a non-finite origin or
- * direction component or a NaN tmin/tmax (tmax = +inf is allowed); n = 0 does nothing. */
+ * direction component, a NaN tmin/tmax (tmax = +inf is allowed) or a negative tmin (the
+ * renderer's rays start at tmin = ray_epsilon >= 0); n = 0 does nothing. */
 trt_status trt_trace_closest(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst);
 trt_status trt_trace_any(trt_context* ctx, const float* rays, uint32_t n, uint8_t* occluded);
 
@@ -308,9 +313,13 @@ trt_status trt_trace_any(trt_context* ctx, const float* rays, uint32_t n, uint8_
 trt_status trt_pick(trt_context* ctx, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
                     trt_pick_result* out);
 
-/* Measurement hook: re-traces the last frame's bounce-0/1 extension and shadow queues
- * with counting kernels and reports average node visits and triangle tests per ray
- * (used for the algorithmic-byte roofline). */
+/* Measurement hook: renders one frame with `opts` through counting variants of the
+ * Extend and Shadow kernels (nothing is accumulated) and reports average node visits
+ * and triangle tests per ray (the bytes every ray loads), how many distinct nodes and
+ * triangle records each launch read (summed over the frame's launches: the bytes a
+ * launch must bring from memory at least; bottom-level trees only in instanced scenes),
+ * and how many traversal-stack entries were moved from the on-chip ring to the global
+ * spill area. */
 typedef struct trt_trace_counts {
   double closest_nodes_per_ray;
   double closest_tris_per_ray;
@@ -320,8 +329,23 @@ typedef struct trt_trace_counts {
   uint64_t any_rays;
   uint32_t node_bytes;
   uint32_t tri_bytes;
+  uint64_t closest_spills;  /* stack entries spilled by the closest-hit traversals */
+  uint64_t any_spills;      /* and by the any-hit traversals */
+  uint64_t closest_distinct_nodes;  /* distinct nodes read per Extend launch, summed over launches */
+  uint64_t closest_distinct_tris;   /* distinct triangle records, likewise */
+  uint64_t any_distinct_nodes;      /* the same for the Shadow launches */
+  uint64_t any_distinct_tris;
+  uint32_t launches;                /* Extend launches of the frame (= Shadow launches) */
 } trt_trace_counts;
 trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* opts, trt_trace_counts* out);
+
+/* Measurement hook (no counterpart in the reference): the vector-memory ceiling of the
+ * traversal's loads. Lanes read 48-byte records (a node or a triangle record, three
+ * 16-byte loads) at pseudo-random indices of a device buffer of `buffer_bytes`, `iters`
+ * records per lane, with independent loads; *gbs = bytes read per second (GB/s, best of
+ * three timed launches). Over a buffer that fits the L2 this is the highest rate such
+ * gathers reach on the device. No context: it runs on `device` with its own buffers. */
+trt_status trt_measure_gather(int device, uint64_t buffer_bytes, uint32_t iters, double* gbs);
 
 /* Differentiable rendering. No counterpart in the reference (its PathTracingKernel,
  * Tracer/CUDA/GPU/Shade/PathTracing.h:8-175, has no adjoint): the adjoint of one frame
@@ -332,7 +356,9 @@ trt_status trt_set_material_params(trt_context* ctx, const float* params, uint32
 /* Gradient of a loss with respect to the evaluation parameters of every material for
  * the last frame rendered by trt_render (same options): dL_dimage = dLoss/d(frame mean
  * image), 3 floats per pixel; out dL_dparams = 5 floats per material. Records are reduced
- * in a fixed order (no float atomics): repeated calls return identical bits. */
+ * in a fixed order (no float atomics): repeated calls return identical bits.
+ * TRT_INVALID_ARG when the options differ from that frame's, or when the scene, camera,
+ * sky or materials changed after it (the replay would not match the frame). */
 trt_status trt_backward(trt_context* ctx, const trt_render_opts* opts, const float* dL_dimage, float* dL_dparams);
 
 /* Learned denoiser (replaces Denoiser, Tracer/Optix/Denoiser.h:14-38 and its

@@ -24,7 +24,9 @@ def build(verbose=False):
         os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread",
         "-ffp-contract=off", "-fno-fast-math", "-march=x86-64-v3",
         "-I", os.path.join(ROOT, "trayracer_amd", "csrc"),
-        os.path.join(HERE, "oracle.cpp"), "-o", tmp,
+        os.path.join(HERE, "oracle.cpp"),
+        # the product's host BVH builder: only the CPU-baseline mode (orc_scene_use_product_bvh) uses it
+        os.path.join(ROOT, "trayracer_amd", "csrc", "bvh_build.cpp"), "-o", tmp,
     ]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -27,6 +27,7 @@
 #include <thread>
 #include <vector>
 
+#include "bvh_build.h"
 #include "trt/camera.h"
 #include "trt/integrator.h"
 #include "trt/render_modes.h"
@@ -145,6 +146,12 @@ struct OScene {
   LightTable lt;
   SceneView view;
   std::vector<Material> samp;  // sampling materials (differentiable renders)
+  bool separate_sampling = false;  // set by orc_set_material_params, as trt_set_material_params
+  // CPU-baseline mode only (orc_scene_use_product_bvh; never used by the parity checks):
+  // the product's binned-SAH tree over pre-split references, traversed near child first
+  bool use_product_bvh = false;
+  std::vector<BuildNode> pnodes;
+  std::vector<uint32_t> porder;  // leaf entries -> triangle
 };
 
 void tri_box_d(const float* p, double* out) {
@@ -182,9 +189,62 @@ inline bool test_tri(const OScene& s, uint32_t off, uint32_t local, const RayTri
   return false;
 }
 
+// Box test of the CPU-baseline traversal: obox's conservative double-precision slab,
+// also returning the entry distance (for near-child-first ordering).
+bool pbox(const BuildNode& n, const double o[3], const double inv[3], double tmin, double tmax, double& tnear) {
+  double t0 = tmin, t1 = tmax;
+  for (int k = 0; k < 3; ++k) {
+    double a = (double(n.lo[k]) - o[k]) * inv[k], b = (double(n.hi[k]) - o[k]) * inv[k];
+    if (a > b) std::swap(a, b);
+    if (std::isnan(a) || std::isnan(b)) {
+      if (o[k] < n.lo[k] || o[k] > n.hi[k]) return false;
+      continue;
+    }
+    t0 = std::max(t0, a);
+    t1 = std::min(t1, b);
+  }
+  tnear = t0;
+  return t0 <= t1 + 1e-6 * (std::fabs(t1) + std::fabs(t0)) + 1e-30;
+}
+
+// CPU-baseline traversal of the product's tree: near child first, fixed stack.
+bool trace_product(const OScene& s, const RayTri& rt, V3 o, V3 d, float tmin, Best& b, bool any) {
+  const double od[3] = {o.x, o.y, o.z};
+  const double inv[3] = {1.0 / double(d.x), 1.0 / double(d.y), 1.0 / double(d.z)};
+  int stack[160];
+  int sp = 0;
+  int node = 0;
+  double t;
+  if (!pbox(s.pnodes[0], od, inv, double(tmin), double(b.t), t)) return false;
+  for (;;) {
+    const BuildNode& n = s.pnodes[size_t(node)];
+    if (n.count > 0) {
+      for (uint32_t i = 0; i < n.count; ++i)
+        if (test_tri(s, 0, s.porder[n.first + i], rt, tmin, b, kNoInstance, any)) return true;
+    } else {
+      double ta, tc;
+      const bool ha = pbox(s.pnodes[size_t(n.left)], od, inv, double(tmin), double(b.t), ta);
+      const bool hc = pbox(s.pnodes[size_t(n.right)], od, inv, double(tmin), double(b.t), tc);
+      if (ha && hc) {
+        const bool lf = ta <= tc;
+        stack[sp++] = lf ? n.right : n.left;
+        node = lf ? n.left : n.right;
+        continue;
+      }
+      if (ha || hc) {
+        node = ha ? n.left : n.right;
+        continue;
+      }
+    }
+    if (sp == 0) return false;
+    node = stack[--sp];
+  }
+}
+
 bool trace_mesh(const OScene& s, uint32_t mesh, V3 o, V3 d, float tmin, Best& b, uint32_t inst, bool any,
                 bool brute) {
   const RayTri rt = ray_tri_setup(o, d);
+  if (s.use_product_bvh && !brute) return trace_product(s, rt, o, d, tmin, b, any);
   const uint32_t off = s.mesh_off[mesh], cnt = s.mesh_cnt[mesh];
   if (brute) {
     for (uint32_t k = 0; k < cnt; ++k)
@@ -383,9 +443,44 @@ void* orc_scene_create(const float* pos, const float* nrm, const uint32_t* mat_i
   return s;
 }
 
+// CPU-baseline mode (single-level scenes): traverse the product's own tree, the binned-SAH
+// BVH over `presplit` pre-split references that the GPU traverses, built by the same host
+// code, instead of the oracle's independent median-split tree. Returns the tree depth, or
+// -1 for an instanced scene or a tree deeper than the fixed traversal stack.
+int orc_scene_use_product_bvh(void* p, float presplit, int threads) {
+  OScene* s = static_cast<OScene*>(p);
+  if (!s->inst.empty()) return -1;
+  const uint32_t n = s->mesh_cnt[0];
+  std::vector<float> box;
+  std::vector<uint32_t> ref_prim;
+  if (presplit > 0.f) {
+    presplit_triangles(s->pos.data(), n, presplit, box, ref_prim);
+  } else {
+    box.resize(size_t(n) * 6);
+    ref_prim.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      const float* q = &s->pos[size_t(i) * 9];
+      for (int k = 0; k < 3; ++k) {
+        box[size_t(i) * 6 + k] = std::min(q[k], std::min(q[3 + k], q[6 + k]));
+        box[size_t(i) * 6 + 3 + k] = std::max(q[k], std::max(q[3 + k], q[6 + k]));
+      }
+      ref_prim[i] = i;
+    }
+  }
+  BuildResult b = build_binned_sah(box.data(), uint32_t(ref_prim.size()), 32, 4, hw_threads(threads),
+                                   kTravCostTriangles);
+  if (b.depth > 150) return -1;
+  s->pnodes = std::move(b.nodes);
+  s->porder.resize(b.prim_order.size());
+  for (size_t k = 0; k < b.prim_order.size(); ++k) s->porder[k] = ref_prim[b.prim_order[k]];
+  s->use_product_bvh = true;
+  return b.depth;
+}
+
 // params: 5 floats per material (base rgb, roughness, metallic); which: 1 eval, 2 sampling, 3 both
 void orc_set_material_params(void* p, const float* params, uint32_t n, uint32_t which) {
   OScene* s = static_cast<OScene*>(p);
+  s->separate_sampling = true;
   for (uint32_t i = 0; i < n && i < s->mats.size(); ++i) {
     for (int pass = 0; pass < 2; ++pass) {
       if (!(which & (1u << pass))) continue;
@@ -474,10 +569,15 @@ void orc_trace_any(void* p, const float* rays, uint32_t n, uint8_t* occ, int mod
 //      bokeh_sides (as float), bokeh_rotation  (15 floats)
 // opts: width, height, spp, max_depth, rr_depth, seed, sample_offset (uint32) +
 //       eps, clamp (float)
+//       mode, mode_flags, crop x0, y0, cw, ch (uint32; cw = 0: the whole film), emitter
+//       strategy (0 MIS, 1 light sampling only, 2 BSDF sampling only)
 // Outputs (nullable): path_rgb [W*H*spp*3] float, image_f32 [W*H*3] (float sum in
 // sample order / spp, as the device Finalize), image_f64 [W*H*3] (FP64 mean),
 // aov [W*H*8] (albedo rgb, pad, normal xyz, view depth) from sample 0,
 // counts [2*max_depth] (extension rays, shadow rays per depth).
+// With a crop, only the paths of the crop's pixels are traced, with the keys and camera
+// rays of the whole film (pixel = y * W + x, path = sample * W * H + pixel); the outputs
+// then cover the crop (cw x ch pixels, rows y0.., samples outermost for path_rgb).
 void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* fopts, float* path_rgb,
                 float* image_f32, double* image_f64, float* aov, uint64_t* counts, int threads) {
   const OScene* s = static_cast<OScene*>(p);
@@ -495,16 +595,25 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
   rp.clamp = fopts[1];
   rp.mode = uopts[7];
   rp.mode_flags = uopts[8];
+  rp.separate_sampling = s->separate_sampling ? 1u : 0u;
+  rp.strategy = uopts[13];
   rp.ao_dist = fopts[2];
   rp.zdepth_max = fopts[3];
-  const uint64_t npix = uint64_t(rp.width) * rp.height;
+  const uint64_t film_pix = uint64_t(rp.width) * rp.height;
+  const bool crop = uopts[11] != 0;
+  const uint32_t cx0 = crop ? uopts[9] : 0u, cy0 = crop ? uopts[10] : 0u;
+  const uint32_t cw = crop ? uopts[11] : rp.width, ch = crop ? uopts[12] : rp.height;
+  const uint64_t npix = uint64_t(cw) * ch;  // pixels of the output (the crop)
   const uint64_t n = npix * rp.spp;
   std::vector<float> L(n * 3, 0.f);
   const uint32_t D = rp.max_depth;
   std::vector<std::atomic<uint64_t>> cnt(size_t(2 * D));
   for (auto& c : cnt) c = 0;
   parallel_for(n, hw_threads(threads), [&](uint64_t pi) {
-    const uint32_t path = uint32_t(pi);
+    const uint64_t smp = pi / npix, opix = pi % npix;  // sample, output pixel
+    const uint64_t film_pixel = uint64_t(cy0 + opix / cw) * rp.width + cx0 + opix % cw;
+    const uint32_t path = uint32_t(smp * film_pix + film_pixel);
+    const bool first_sample = path < film_pix;
     PathState ps;
     V3 acc = v3(0.f);
     std::vector<uint64_t> local(2 * D, 0);
@@ -517,8 +626,8 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
         ShadeOut out;
         shade_mode(s->view, s->samp.data(), rp, ps, h, d, out);
         if (!is_black(out.L)) acc = acc + out.L;
-        if (d == 0 && path < npix && aov) {
-          float* a = aov + size_t(path) * 8;
+        if (d == 0 && first_sample && aov) {
+          float* a = aov + size_t(opix) * 8;
           a[0] = out.albedo.x; a[1] = out.albedo.y; a[2] = out.albedo.z; a[3] = 1.f;
           a[4] = out.normal.x; a[5] = out.normal.y; a[6] = out.normal.z; a[7] = out.vdepth;
         }
@@ -530,8 +639,8 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
         if (!out.has_ext) break;
         ps = out.next;
       }
-    } else if (path < npix && aov) {
-      std::memset(aov + size_t(path) * 8, 0, 32);
+    } else if (first_sample && aov) {
+      std::memset(aov + size_t(opix) * 8, 0, 32);
     }
     L[pi * 3 + 0] = acc.x;
     L[pi * 3 + 1] = acc.y;
@@ -582,6 +691,7 @@ void orc_backward(void* p, const float* cam, const uint32_t* uopts, const float*
   rp.sample_offset = uopts[6];
   rp.eps = fopts[0];
   rp.clamp = fopts[1];
+  rp.separate_sampling = s->separate_sampling ? 1u : 0u;
   const uint64_t npix = uint64_t(rp.width) * rp.height;
   const uint64_t n = npix * rp.spp;
   const uint32_t D = rp.max_depth;
@@ -793,6 +903,41 @@ void orc_intersect(const float* rays, const float* tris, uint32_t n, float* out)
     out[4 * i + 3] = h.b2;
   }
 }
+// anisotropic GGX distribution D(m) at n unit half vectors (3 floats each)
+void orc_ggx_d(float ax, float ay, const float* m, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = ggx_d(v3(m[3 * i], m[3 * i + 1], m[3 * i + 2]), ax, ay);
+}
+
+// GTR1 (clearcoat) distribution D(m) at n unit half vectors
+void orc_gtr1_d(float a, const float* m, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = gtr1_d(v3(m[3 * i], m[3 * i + 1], m[3 * i + 2]), a);
+}
+
+// Surface attributes at each ray's closest hit (surface_at, as the shading step reads
+// them): out 11 floats per ray = hit point (3), geometric normal of the world-space
+// triangle (3), shading normal facing the ray (3), world-space area, material id (bits);
+// a miss gives zeros and material 0xffffffff.
+void orc_surface(void* p, const float* rays, uint32_t n, float* out) {
+  const OScene* s = static_cast<OScene*>(p);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* r = rays + size_t(i) * 8;
+    float* o = out + size_t(i) * 11;
+    std::memset(o, 0, 11 * 4);
+    uint32_t mat = kNoHit;
+    Best b;
+    trace(*s, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7], b, false, false);
+    if (b.prim != kNoHit) {
+      const Hit h{b.t, b.b1, b.b2, b.prim, b.inst};
+      const SurfacePoint sp = surface_at(s->view, h, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]));
+      const float v[10] = {sp.p.x, sp.p.y, sp.p.z, sp.ng_raw.x, sp.ng_raw.y, sp.ng_raw.z, sp.ns.x, sp.ns.y, sp.ns.z,
+                           sp.area};
+      std::memcpy(o, v, 40);
+      mat = sp.mat;
+    }
+    std::memcpy(o + 10, &mat, 4);
+  }
+}
+
 // camera rays for explicit samples: u = 5 per ray (jx, jy, l0, l1, l2); px = x,y
 void orc_camera_rays(const float* cam, uint32_t W, uint32_t H, const uint32_t* px, const float* u, uint32_t n,
                      float* out /* 7: o, d, weight */) {

@@ -18,6 +18,7 @@ _PROTO = [
     ("orc_scene_create", _P, [_f, _f, _u, ctypes.c_uint32, _u, _u, ctypes.c_uint32, _f, ctypes.c_uint32, _f,
                               ctypes.c_uint32]),
     ("orc_scene_destroy", None, [_P]),
+    ("orc_scene_use_product_bvh", ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int]),
     ("orc_light_count", ctypes.c_uint32, [_P]),
     ("orc_total_power", ctypes.c_float, [_P]),
     ("orc_light_table", None, [_P, _f, _f, _u]),
@@ -47,6 +48,9 @@ _PROTO = [
     ("orc_bsdf_sample", None, [_f, _f, _f, ctypes.c_uint32, _f, _f]),
     ("orc_intersect", None, [_f, _f, ctypes.c_uint32, _f]),
     ("orc_camera_rays", None, [_f, ctypes.c_uint32, ctypes.c_uint32, _u, _f, ctypes.c_uint32, _f]),
+    ("orc_ggx_d", None, [ctypes.c_float, ctypes.c_float, _f, ctypes.c_uint32, _f]),
+    ("orc_gtr1_d", None, [ctypes.c_float, _f, ctypes.c_uint32, _f]),
+    ("orc_surface", None, [_P, _f, ctypes.c_uint32, _f]),
 ]
 
 _lib = None
@@ -124,6 +128,21 @@ class Oracle:
         self.lib.orc_sky_radiance(self.handle, fp(dirs), dirs.shape[0], int(sun), fp(out))
         return out
 
+    def use_product_bvh(self, presplit=6.0, threads=0):
+        """CPU-baseline mode: trace through the product's own binned-SAH tree (pre-split,
+        as the GPU does) instead of the oracle's independent tree. Returns its depth."""
+        d = self.lib.orc_scene_use_product_bvh(self.handle, float(presplit), threads)
+        if d < 0:
+            raise ValueError("the product-tree mode needs a single-level scene")
+        return d
+
+    def surface(self, rays):
+        """Surface attributes at each ray's closest hit: dict(p, ng, ns, area, mat)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.empty((rays.shape[0], 11), np.float32)
+        self.lib.orc_surface(self.handle, fp(rays), rays.shape[0], fp(out))
+        return dict(p=out[:, 0:3], ng=out[:, 3:6], ns=out[:, 6:9], area=out[:, 9], mat=out[:, 10].view(np.uint32))
+
     def close(self):
         if self.handle:
             self.lib.orc_scene_destroy(self.handle)
@@ -159,16 +178,24 @@ class Oracle:
             self.lib.orc_light_table(self.handle, fp(L), fp(p), up(a))
         return L, p, a, float(self.lib.orc_total_power(self.handle))
 
-    def render(self, render=None, camera=None, sample_offset=0, threads=0):
+    def render(self, render=None, camera=None, sample_offset=0, threads=0, crop=None):
         """Returns dict(path_rgb [N,3] f32, image_f32 [H,W,3], image_f64 [H,W,3],
-        aov [H,W,8], counts [2, max_depth])."""
+        aov [H,W,8], counts [2, max_depth]). crop = (x0, y0, w, h): trace only that
+        window of the film (same keys and camera rays as the whole film); the outputs
+        then cover the window, path_rgb as [spp][h][w]. render may hold "strategy": 1 =
+        light sampling only, 2 = BSDF sampling only (0, both with MIS, is the renderer's)."""
         r = dict(self.scene["render"])
         if render:
             r.update(render)
         cam = cam_array(camera or self.scene["camera"])
         W, H, spp, D = r["width"], r["height"], r["spp"], r["max_depth"]
+        x0, y0, cw, ch = crop if crop is not None else (0, 0, 0, 0)
+        if crop is not None:
+            assert cw > 0 and ch > 0 and x0 + cw <= W and y0 + ch <= H, crop
         uo = np.array([W, H, spp, D, r["rr_depth"], r["seed"], sample_offset, r.get("mode", 0),
-                       r.get("mode_flags", 0)], np.uint32)
+                       r.get("mode_flags", 0), x0, y0, cw, ch, r.get("strategy", 0)], np.uint32)
+        if crop is not None:
+            W, H = cw, ch
         fo = np.array([r["ray_epsilon"], r["clamp"], r.get("ao_dist", 1.0), r.get("zdepth_max", 1.0)], np.float32)
         paths = np.empty((W * H * spp, 3), np.float32)
         img32 = np.empty((H, W, 3), np.float32)
@@ -331,4 +358,18 @@ def tone_curve(x, method):
     x = np.ascontiguousarray(x, np.float32)
     out = np.empty_like(x)
     load_oracle().orc_tone_curve(fp(x), x.size, method, fp(out))
+    return out
+
+
+def ggx_d(ax, ay, m):
+    m = np.ascontiguousarray(m, np.float32).reshape(-1, 3)
+    out = np.empty(m.shape[0], np.float32)
+    load_oracle().orc_ggx_d(ax, ay, fp(m), m.shape[0], fp(out))
+    return out
+
+
+def gtr1_d(a, m):
+    m = np.ascontiguousarray(m, np.float32).reshape(-1, 3)
+    out = np.empty(m.shape[0], np.float32)
+    load_oracle().orc_gtr1_d(a, fp(m), m.shape[0], fp(out))
     return out

@@ -8,6 +8,7 @@ smooth function of the evaluation parameters and central differences (eps = 1e-3
 with identical random numbers converge to its exact derivative.
 """
 import numpy as np
+import pytest
 
 from oracle import oracle_py as O
 from trayracer_amd import scenes
@@ -18,8 +19,12 @@ def _loss_and_grad_image(img, target_img):
     return float((d * d).sum()), (2.0 * d).astype(np.float32)
 
 
-def test_adjoint_matches_central_differences():
-    sc, target = scenes.cornell_diff(width=24, height=24, spp=8, max_depth=4, seed=3)
+@pytest.mark.parametrize("rr_depth", [None, 1], ids=["rr-last-vertex", "rr-from-depth-1"])
+def test_adjoint_matches_central_differences(rr_depth):
+    """With Russian roulette from depth 1 (PathTracing.h:148-154) the survival
+    probability is a sampling decision: held fixed with the sampling parameters, the
+    continuation derivative stays exact."""
+    sc, target = scenes.cornell_diff(width=24, height=24, spp=8, max_depth=4, seed=3, rr_depth=rr_depth)
     orc = O.Oracle(sc)
     theta = scenes.material_params(sc["materials"])
     orc.set_material_params(target, which=3)

@@ -163,3 +163,90 @@ def test_white_metal_albedo(roughness):
         if theta == 0.0:
             expect = _ggx_albedo_normal_incidence(roughness ** 2)
             assert abs(albedo - expect) < 4 * se + 2e-3, (albedo, expect)
+
+
+def _black(**kw):
+    """Disney material with no diffuse, no specular reflectance at normal incidence and
+    no other lobe unless given: base colour black, specular 0, metallic 0."""
+    args = dict(roughness=0.5, metallic=0.0, specular=0.0, sheen=0.0, clearcoat=0.0)
+    args.update(kw)
+    return material((0.0, 0.0, 0.0), **args)
+
+
+@pytest.mark.parametrize("gloss", [0.0, 0.4, 1.0])
+def test_clearcoat_normal_incidence(gloss):
+    """Burley 2012 clearcoat at wo = wi = n: 0.25 c D_GTR1(1) F(0.04) G(1)^2 / 4 with
+    D_GTR1(cos 0) = (a^2 - 1) / (pi ln(a^2) a^2), a = lerp(0.1, 0.001, gloss), and the
+    Smith factors equal to 1; the other lobes vanish for this material."""
+    cc = 0.7
+    z = np.array([[0, 0, 1]], np.float32)
+    f, _ = O.bsdf_eval(_black(clearcoat=cc, clearcoat_gloss=gloss), z, z)
+    a = 0.1 + (0.001 - 0.1) * gloss
+    d = (a * a - 1) / (math.pi * math.log(a * a) * a * a)
+    np.testing.assert_allclose(f[0], 0.25 * cc * d * 0.04 / 4, rtol=2e-5)
+
+
+def _hemisphere_grid(nt=1500, nphi=720):
+    """Half vectors on a (theta, phi) grid dense near the pole, with quadrature weights
+    (theta = pi/2 u^3)."""
+    u = (np.arange(nt) + 0.5) / nt
+    th = 0.5 * math.pi * u ** 3
+    jac = 0.5 * math.pi * 3 * u ** 2
+    phi = (np.arange(nphi) + 0.5) / nphi * 2 * math.pi
+    T, P = np.meshgrid(th, phi, indexing="ij")
+    m = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], axis=-1).reshape(-1, 3)
+    w = (np.sin(T) * jac[:, None]).reshape(-1) / nt * (2 * math.pi / nphi)
+    return m, np.cos(T).reshape(-1), w
+
+
+@pytest.mark.parametrize("a", [0.1, 0.03, 0.001])
+def test_gtr1_is_normalised(a):
+    """The clearcoat distribution integrates to one against cos(theta_m) over the
+    hemisphere (quadrature of the oracle's own D); at a = 0.001 this needs its
+    denominator formed without cancellation."""
+    m, ct, w = _hemisphere_grid(nt=4000 if a < 0.01 else 1500)
+    D = O.gtr1_d(a, m).astype(np.float64)
+    assert abs((D * ct * w).sum() - 1.0) < 2e-3
+
+
+def test_sheen_at_known_half_angle():
+    """Burley 2012 sheen = sheen x C_sheen x (1 - cos theta_d)^5 (x (1 - metallic)),
+    C_sheen = 1 for sheen tint 0, theta_d the angle between wi and the half vector; with
+    a near-zero roughness the specular lobe is negligible away from the mirror
+    direction, and the diffuse and clearcoat lobes vanish."""
+    sheen = 0.8
+    m = _black(sheen=sheen, sheen_tint=0.0, roughness=0.0)
+    wo = np.array([[math.sin(1.1), 0.0, math.cos(1.1)]], np.float32)
+    wi = np.array([[0.0, math.sin(0.9), math.cos(0.9)]], np.float32)
+    f, _ = O.bsdf_eval(m, wo, wi)
+    h = (wo[0].astype(np.float64) + wi[0]) / np.linalg.norm(wo[0].astype(np.float64) + wi[0])
+    cd = float(np.dot(wi[0], h))
+    np.testing.assert_allclose(f[0], sheen * (1.0 - cd) ** 5, rtol=1e-5)
+    # metallic 0.5 halves it
+    f2, _ = O.bsdf_eval(_black(sheen=sheen, sheen_tint=0.0, roughness=0.0, metallic=0.5), wo, wi)
+    np.testing.assert_allclose(f2[0], 0.5 * sheen * (1.0 - cd) ** 5, rtol=1e-5)
+
+
+@pytest.mark.parametrize("sub", [0.0, 0.4, 1.0])
+def test_subsurface_normal_incidence(sub):
+    """At wo = wi = n Burley's diffuse is base / pi and the Hanrahan-Krueger subsurface
+    term 1.25 (F_ss (1 / (cos_i + cos_o) - 0.5) + 0.5) base / pi with F_ss = 1 is
+    0.625 base / pi; the BRDF blends them by `subsurface` (specular 0: no other lobe)."""
+    base = np.array([0.5, 0.3, 0.2])
+    m = material(tuple(base), roughness=0.6, metallic=0.0, specular=0.0, subsurface=sub)
+    z = np.array([[0, 0, 1]], np.float32)
+    f, _ = O.bsdf_eval(m, z, z)
+    np.testing.assert_allclose(f[0], base / math.pi * ((1 - sub) + 0.625 * sub), rtol=2e-6)
+
+
+@pytest.mark.parametrize("ax,ay", [(0.3, 0.08), (0.5, 0.5), (0.05, 0.4)])
+def test_anisotropic_ggx_is_normalised(ax, ay):
+    """The anisotropic GGX distribution integrates to one against cos(theta_m) over the
+    hemisphere (quadrature of the oracle's own D on a grid dense near the peak)."""
+    m, ct, w = _hemisphere_grid()
+    D = O.ggx_d(ax, ay, m).astype(np.float64)
+    integral = (D * ct * w).sum()
+    assert abs(integral - 1.0) < 2e-3, integral
+    # the aspect: D falls off faster along the axis of the smaller alpha
+    d = O.ggx_d(ax, ay, np.array([[math.sin(0.1), 0, math.cos(0.1)], [0, math.sin(0.1), math.cos(0.1)]]))
+    assert (d[0] > d[1]) == (ax > ay) or ax == ay

@@ -10,6 +10,7 @@
 import math
 
 import numpy as np
+import pytest
 
 from oracle import oracle_py as O
 from tests.gpu_util import prim_of, random_rays
@@ -146,3 +147,38 @@ def test_camera_thin_lens_focus():
     assert np.ptp(p[:, 0]) < 1e-5 and np.ptp(p[:, 1]) < 1e-5
     # lens points stay inside the hexagon of circumradius `aperture`
     assert np.max(np.linalg.norm(o[:, 0:2], axis=1)) <= 0.1 + 1e-6
+
+
+@pytest.mark.parametrize("distortion", [0.0, 0.5, 1.2])
+def test_camera_barrel_distortion_closed_form(distortion):
+    """Barrel distortion (Tracer/Optix/GPU/optix.cu:117-138): a film point at lens radius
+    r = tan(fov/2) x |screen| leaves at the angle arctan(tan(r d k) / d) to the forward axis
+    (k = atan(t d / 2) / (t d / 2), t = tan(fov/2)). At the horizontal film edge (r = t)
+    the double-angle identity gives tan(angle) = t / (1 - (t d / 2)^2), independent of the
+    formula's evaluation order; without distortion the angle is arctan(r)."""
+    fov = math.radians(70.0)
+    t = math.tan(fov / 2)
+    cam = dict(position=(0.0, 0.0, 0.0), target=(0.0, 0.0, 1.0), up=(0, 1, 0), fov=fov, distortion=distortion)
+    W, H = 64, 64
+    # right edge of the middle row: screen = (1, 0)
+    r = O.camera_rays(cam, W, H, [[W - 1, H // 2]], [[1.0, 0.0, 0.5, 0.5, 0.0]])[0]
+    tan_angle = math.hypot(r[3], r[4]) / r[5]
+    want = t if distortion == 0.0 else t / (1.0 - (t * distortion / 2) ** 2)
+    assert abs(tan_angle - want) < 2e-6 * want
+    # interior points: radial scale tan(r d k) / (r d), direction kept
+    rng = np.random.default_rng(2)
+    px = np.stack([rng.integers(0, W, 200), rng.integers(0, H, 200)], axis=1).astype(np.uint32)
+    jit = rng.random((200, 2))
+    out = O.camera_rays(cam, W, H, px, np.concatenate([jit, np.full((200, 3), 0.5)], axis=1))
+    sx = ((px[:, 0] + jit[:, 0]) / W) * 2 - 1
+    sy = ((px[:, 1] + jit[:, 1]) / H) * 2 - 1
+    lx, ly = t * sx, t * sy
+    rad = np.hypot(lx, ly)
+    if distortion > 0:
+        d = t * distortion / 2
+        k = np.tan(rad * distortion * math.atan(d) / d) / (rad * distortion)
+    else:
+        k = np.ones_like(rad)
+    tx, ty = -out[:, 3] / out[:, 5], out[:, 4] / out[:, 5]  # side = -x for this camera
+    np.testing.assert_allclose(np.hypot(tx, ty), rad * k, rtol=2e-5)
+    np.testing.assert_allclose(np.arctan2(ty, tx), np.arctan2(ly, lx), atol=2e-5)

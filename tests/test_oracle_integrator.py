@@ -106,3 +106,39 @@ def test_direct_view_of_emitter_is_exact():
     out = O.Oracle(sc).render()
     vig = _vignetting(sc).astype(np.float32)
     np.testing.assert_array_equal(out["path_rgb"][:, 0], (vig * np.float32(le)).astype(np.float32))
+
+
+def test_render_crop_equals_window_of_full_frame():
+    """A cropped oracle render traces the crop's paths with the whole film's keys and
+    camera: its per-path radiance, images and AOVs equal the same window of a full render
+    (the full-size GPU parity tests compare a window of a full-size GPU frame this way)."""
+    from oracle import Oracle
+    from trayracer_amd import scenes
+    sc = scenes.cornell(width=40, height=30, spp=3, max_depth=4, seed=4)
+    orc = Oracle(sc)
+    full = orc.render()
+    x0, y0, w, h = 7, 11, 13, 9
+    cut = orc.render(crop=(x0, y0, w, h))
+    paths = full["path_rgb"].reshape(3, 30, 40, 3)[:, y0:y0 + h, x0:x0 + w].reshape(-1, 3)
+    assert np.array_equal(cut["path_rgb"], paths)
+    for k in ("image_f32", "image_f64", "aov"):
+        assert np.array_equal(cut[k], full[k][y0:y0 + h, x0:x0 + w]), k
+    assert cut["counts"].sum() < full["counts"].sum()
+
+
+def test_config0_reference_fixture_is_the_oracle_image():
+    """The committed FP64 4096-spp Cornell image (tests/golden/config0_ref4096.npz) is the
+    oracle's: windows at a corner, at the light and at the metal box re-rendered here."""
+    import os
+
+    from oracle import Oracle
+    from trayracer_amd import scenes
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_ref4096.npz"))
+    sc = scenes.cornell()
+    assert (int(d["width"]), int(d["height"]), int(d["max_depth"]), int(d["rr_depth"]), int(d["seed"])) == \
+        (256, 256, 4, 3, 0)
+    orc = Oracle(sc)
+    for x0, y0 in ((0, 0), (126, 230), (150, 60)):
+        out = orc.render(render=dict(spp=int(d["spp"])), crop=(x0, y0, 3, 3))
+        want = d["mean"][y0:y0 + 3, x0:x0 + 3]
+        assert np.array_equal(out["image_f64"].astype(np.float32), want), (x0, y0)

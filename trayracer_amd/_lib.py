@@ -131,6 +131,10 @@ class TrtTraceCounts(ctypes.Structure):
         ("any_nodes_per_ray", ctypes.c_double), ("any_tris_per_ray", ctypes.c_double),
         ("closest_rays", ctypes.c_uint64), ("any_rays", ctypes.c_uint64),
         ("node_bytes", ctypes.c_uint32), ("tri_bytes", ctypes.c_uint32),
+        ("closest_spills", ctypes.c_uint64), ("any_spills", ctypes.c_uint64),
+        ("closest_distinct_nodes", ctypes.c_uint64), ("closest_distinct_tris", ctypes.c_uint64),
+        ("any_distinct_nodes", ctypes.c_uint64), ("any_distinct_tris", ctypes.c_uint64),
+        ("launches", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -160,6 +164,7 @@ PROTOTYPES = [
     ("trt_render", S, [P, ctypes.POINTER(TrtRenderOpts), ctypes.POINTER(TrtStats)]),
     ("trt_reset", S, [P]),
     ("trt_sample_count", ctypes.c_uint32, [P]),
+    ("trt_frame_id", ctypes.c_uint64, [P]),
     ("trt_read_accum", S, [P, c_float_p]),
     ("trt_read_color", S, [P, c_float_p]),
     ("trt_read_albedo", S, [P, c_float_p]),
@@ -170,6 +175,7 @@ PROTOTYPES = [
     ("trt_pick", S, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                      ctypes.POINTER(TrtPickResult)]),
     ("trt_measure_traversal", S, [P, ctypes.POINTER(TrtRenderOpts), ctypes.POINTER(TrtTraceCounts)]),
+    ("trt_measure_gather", S, [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     ("trt_set_material_params", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("trt_backward", S, [P, ctypes.POINTER(TrtRenderOpts), c_float_p, c_float_p]),
     ("trt_denoiser_init", S, [P, ctypes.c_uint32]),

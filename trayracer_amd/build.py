@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrt.so")
 SOURCES = ["api.cpp", "bvh_build.cpp", "kernels.hip", "denoiser.cpp", "denoiser_kernels.hip", "scene_file.cpp",
-           "scene_file_api.cpp", "image_io.cpp"]
+           "scene_file_api.cpp", "image_io.cpp", "probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TRT_ARCH", "gfx950")
 

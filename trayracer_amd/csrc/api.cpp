@@ -50,6 +50,7 @@ struct DevBuf {
     if (n == 0) return hipSuccess;
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
+      (void)hipGetLastError();  // reported here: later launch checks must not see it again
       p = nullptr;
       return e;
     }
@@ -88,6 +89,8 @@ struct trt_context {
   DevBuf d_pos, d_nrm, d_mat_ids, d_materials, d_inst, d_inst_tlas, d_lights, d_alias_p, d_alias_i;
   DevBuf d_nodes, d_tris, d_tlas;
   DevBuf d_spill;  // traversal stack spill areas
+  DevBuf d_touch;  // counting frames: touch bitmaps of Extend (nodes, triangles), then Shadow's
+  uint32_t touch_node_words = 0, touch_tri_words = 0;
   uint32_t n_lights = 0;
   float total_power = 0.f;
   bool use_normals = false;
@@ -129,7 +132,8 @@ struct trt_context {
   // differentiable rendering
   std::vector<Material> sampling_materials;  // empty = same as materials
   DevBuf d_mat_sampling;
-  bool has_frame = false;
+  bool has_frame = false;      // the last frame's per-path radiance matches the current scene
+  uint64_t frame_id = 0;       // frames rendered by this context
   trt_render_opts last_opts{};
   uint32_t last_sample_offset = 0;
   DevBuf a_lrem, a_pend, a_key, a_g, a_sga, a_sgb, a_dldi, a_partial, a_out;
@@ -142,6 +146,10 @@ struct trt_context {
 };
 
 namespace {
+
+// Scene, camera or material changes make the last frame's per-path radiance stale for
+// the adjoint, which replays that frame against the current state.
+void invalidate_frame(trt_context* c) { c->has_frame = false; }
 
 trt_status fail(trt_context* c, trt_status s, const std::string& msg) {
   if (c) c->err = msg;
@@ -296,6 +304,8 @@ DevAccel accel_of(const trt_context* ctx) {
   a.stack_cap = ctx->stack_cap;
   a.two_level = ctx->two_level ? 1 : 0;
   a.spill = ctx->d_spill.as<uint32_t>();
+  a.touch_nodes = nullptr;
+  a.touch_tris = nullptr;
   return a;
 }
 
@@ -337,6 +347,7 @@ RenderParams params_of_opts(const trt_context* ctx, const trt_render_opts& o) {
   rp.mode_flags = o.mode_flags;
   rp.ao_dist = o.ao_dist;
   rp.zdepth_max = o.zdepth_max;
+  rp.separate_sampling = ctx->sampling_materials.empty() ? 0u : 1u;
   return rp;
 }
 
@@ -346,7 +357,7 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
   if (s != TRT_OK) return s;
   const RenderParams rp = params_of_opts(ctx, o);
   const uint32_t n = o.width * o.height * o.spp;
-  const DevAccel acc = accel_of(ctx);
+  const DevAccel acc0 = accel_of(ctx);
   const SceneView sv = view_of(ctx);
   uint32_t* ctr = ctx->counters.as<uint32_t>();
   FrameBuffers fb;
@@ -373,8 +384,21 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
 
   const uint32_t cap = ctx->seg_cap;
   // the concurrent Shadow launches use the second spill area
+  DevAccel acc = acc0;
   DevAccel acc_sh = acc;
   acc_sh.spill = acc.spill + ctx->spill_half;
+  // counting frames: every launch marks the nodes and triangle records it reads
+  const uint32_t tw = ctx->touch_node_words + ctx->touch_tri_words;
+  unsigned long long* ctr64 = reinterpret_cast<unsigned long long*>(ctr + kCtrStats);
+  if (stats) {
+    HIPC(ctx->d_touch.ensure(size_t(2) * tw * 4), "alloc touch bitmaps");
+    HIPC(hipMemsetAsync(ctx->d_touch.p, 0, size_t(2) * tw * 4, stm), "clear touch bitmaps");
+    uint32_t* t = ctx->d_touch.as<uint32_t>();
+    acc.touch_nodes = t;
+    acc.touch_tris = t + ctx->touch_node_words;
+    acc_sh.touch_nodes = t + tw;
+    acc_sh.touch_tris = t + tw + ctx->touch_node_words;
+  }
   hipStream_t stm2 = ctx->stream2;
   hipEvent_t* ev2 = ctx->ev2;
 
@@ -387,6 +411,10 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
     const int di = int(d);
     launch_extend(acc, q[cur], hb, SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkExt, di), cap}, ctr,
                   ctx->grid_trace, stats, stm);
+    if (stats) {
+      launch_touch_count(acc.touch_nodes, ctx->touch_node_words, ctr64 + 6, stm);
+      launch_touch_count(acc.touch_tris, ctx->touch_tri_words, ctr64 + 7, stm);
+    }
     HIPC(hipEventRecord(ev[4 + 4 * d + 0], stm), "event");
     // Shade adds to the radiance that the previous bounce's Shadow adds to, and reuses
     // its shadow queue: wait for it
@@ -401,6 +429,10 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
     launch_shadow(acc_sh, sq, fb.radiance,
                   SegIn{ctr + seg_ctr(kSegShadow, di), ctr + seg_ctr(kSegWorkShadow, di), cap}, ctr, ctx->grid_trace,
                   stats, stm2);
+    if (stats) {
+      launch_touch_count(acc_sh.touch_nodes, ctx->touch_node_words, ctr64 + 8, stm2);
+      launch_touch_count(acc_sh.touch_tris, ctx->touch_tri_words, ctr64 + 9, stm2);
+    }
     HIPC(hipEventRecord(ev2[2 * d + 1], stm2), "event");
   }
   if (D > 0) HIPC(hipStreamWaitEvent(stm, ev2[2 * (D - 1) + 1], 0), "wait");
@@ -415,6 +447,7 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
   HIPC(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost), "read counters");
   ctx->last_paths = n;
   ctx->has_frame = true;
+  ++ctx->frame_id;
   ctx->last_opts = o;
   ctx->last_sample_offset = rp.sample_offset;
   if (accumulate) ctx->sample_count += o.spp;
@@ -523,6 +556,7 @@ trt_status trt_clear_scene(trt_context* ctx) {
   ctx->sky = SkyParams{};
   ctx->built = false;
   ctx->dirty = true;
+  invalidate_frame(ctx);
   return TRT_OK;
 }
 
@@ -534,6 +568,7 @@ trt_status trt_set_sky(trt_context* ctx, const trt_sky* sky) {
   ctx->sky = make_sky(sky->enabled != 0, sky->draw_sun != 0, sky->sun_dir, sky->angular_diameter, sky->intensity,
                       sky->turbidity);
   ctx->sample_count = 0;
+  invalidate_frame(ctx);
   return TRT_OK;
 }
 
@@ -551,6 +586,7 @@ trt_status trt_set_materials(trt_context* ctx, const trt_material* mats, uint32_
   ctx->materials = std::move(m);
   ctx->sampling_materials.clear();
   ctx->built = false;
+  invalidate_frame(ctx);
   return TRT_OK;
 }
 
@@ -575,6 +611,7 @@ trt_status trt_add_mesh(trt_context* ctx, const float* positions, const float* n
   ctx->meshes.push_back(m);
   if (mesh_id) *mesh_id = uint32_t(ctx->meshes.size() - 1);
   ctx->built = false;
+  invalidate_frame(ctx);
   return TRT_OK;
 }
 
@@ -585,6 +622,7 @@ trt_status trt_set_instances(trt_context* ctx, const trt_instance* inst, uint32_
     if (!finite_all(inst[i].transform, 12)) return fail(ctx, TRT_INVALID_ARG, "non-finite instance transform");
   ctx->instances_in.assign(inst, inst + n);
   ctx->built = false;
+  invalidate_frame(ctx);
   return TRT_OK;
 }
 
@@ -764,6 +802,8 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   // stack spill areas of the traversals: stack_cap entries per grid thread, twice (Extend
   // and the Shadow launch that runs beside it)
   ctx->spill_half = size_t(ctx->grid_trace) * 256 * size_t(ctx->stack_cap);
+  ctx->touch_node_words = uint32_t((nodes48.size() + 31) / 32);
+  ctx->touch_tri_words = uint32_t((tris.size() / 12 + 31) / 32);
   HIPC(ctx->d_spill.ensure(2 * ctx->spill_half * 4), "alloc stack spill");
 
   const auto t1 = std::chrono::steady_clock::now();
@@ -780,6 +820,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   ctx->info.references = n_refs;
   ctx->built = true;
   ctx->sample_count = 0;
+  invalidate_frame(ctx);
   return TRT_OK;
 }
 
@@ -803,6 +844,7 @@ trt_status trt_set_camera(trt_context* ctx, const trt_camera* c) {
   ctx->cam = make_camera(p, t, u, c->fov, c->aperture, c->focal_dist, c->distortion, c->bokeh_sides, c->bokeh_rotation);
   ctx->camera_set = true;
   ctx->sample_count = 0;  // the reference resets accumulation on camera change
+  invalidate_frame(ctx);
   return TRT_OK;
 }
 
@@ -841,7 +883,7 @@ trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* o, trt
   trt_stats st;
   s = run_frame(ctx, *o, true, false, &st);
   if (s != TRT_OK) return s;
-  unsigned long long c[4];
+  unsigned long long c[10];
   HIPC(hipMemcpy(c, ctx->counters.as<uint32_t>() + kCtrStats, sizeof(c), hipMemcpyDeviceToHost), "read stats");
   std::memset(out, 0, sizeof(*out));
   out->closest_rays = st.extend_rays;
@@ -852,6 +894,13 @@ trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* o, trt
   out->any_tris_per_ray = st.shadow_rays ? double(c[3]) / st.shadow_rays : 0.0;
   out->node_bytes = sizeof(DevNode48);
   out->tri_bytes = 48;
+  out->closest_spills = c[4];
+  out->any_spills = c[5];
+  out->closest_distinct_nodes = c[6];
+  out->closest_distinct_tris = c[7];
+  out->any_distinct_nodes = c[8];
+  out->any_distinct_tris = c[9];
+  out->launches = st.extend_launches;
   return TRT_OK;
 }
 
@@ -862,6 +911,8 @@ trt_status trt_reset(trt_context* ctx) {
 }
 
 uint32_t trt_sample_count(const trt_context* ctx) { return ctx ? ctx->sample_count : 0u; }
+
+uint64_t trt_frame_id(const trt_context* ctx) { return ctx ? ctx->frame_id : 0u; }
 
 static trt_status read_pixels(trt_context* ctx, DevBuf& b, float* out) {
   if (!ctx || !out) return TRT_INVALID_ARG;
@@ -907,9 +958,12 @@ static trt_status trace_common(trt_context* ctx, const float* rays, uint32_t n, 
   std::vector<float> soa(size_t(n) * 8);
   for (uint32_t i = 0; i < n; ++i) {
     const float* r = &rays[size_t(i) * 8];
-    // origins and directions finite, interval bounds not NaN (tmax may be +inf)
+    // origins and directions finite, interval bounds not NaN (tmax may be +inf), tmin >= 0
+    // (the traversal's box test widens far distances relatively, which is conservative
+    // only for boxes at non-negative distances)
     if (!finite_all(r, 3) || !finite_all(r + 4, 3) || r[3] != r[3] || r[7] != r[7])
       return fail(ctx, TRT_INVALID_ARG, "ray " + std::to_string(i) + " has a non-finite origin or direction or a NaN bound");
+    if (!(r[3] >= 0.f)) return fail(ctx, TRT_INVALID_ARG, "ray " + std::to_string(i) + " has a negative tmin");
     std::memcpy(&soa[size_t(i) * 4], &rays[size_t(i) * 8], 16);
     std::memcpy(&soa[(size_t(n) + i) * 4], &rays[size_t(i) * 8 + 4], 16);
   }
@@ -1021,6 +1075,7 @@ trt_status trt_set_material_params(trt_context* ctx, const float* params, uint32
        "upload sampling materials");
   HIPC(hipStreamSynchronize(ctx->stream), "upload materials");
   ctx->sample_count = 0;
+  invalidate_frame(ctx);
   return TRT_OK;
 }
 
@@ -1029,9 +1084,13 @@ trt_status trt_backward(trt_context* ctx, const trt_render_opts* o, const float*
   if (s != TRT_OK) return s;
   if (!dldi || !grad) return TRT_INVALID_ARG;
   const trt_render_opts& L = ctx->last_opts;
-  if (!ctx->has_frame || L.width != o->width || L.height != o->height || L.spp != o->spp ||
-      L.max_depth != o->max_depth || L.rr_depth != o->rr_depth || L.seed != o->seed ||
-      L.ray_epsilon != o->ray_epsilon || L.clamp != o->clamp)
+  if (!ctx->has_frame)
+    return fail(ctx, TRT_INVALID_ARG,
+                "trt_backward replays the last trt_render, and the scene, camera, sky or materials changed since "
+                "(or nothing was rendered): render the frame again first");
+  if (L.width != o->width || L.height != o->height || L.spp != o->spp || L.max_depth != o->max_depth ||
+      L.rr_depth != o->rr_depth || L.seed != o->seed || L.ray_epsilon != o->ray_epsilon || L.clamp != o->clamp ||
+      L.mode != o->mode)
     return fail(ctx, TRT_INVALID_ARG, "trt_backward replays the last trt_render: call it with the same options");
   if (o->clamp != 0.f) return fail(ctx, TRT_UNSUPPORTED, "the adjoint needs clamp = 0");
   if (o->mode != kModePathTracing) return fail(ctx, TRT_UNSUPPORTED, "the adjoint needs the path-tracing mode");

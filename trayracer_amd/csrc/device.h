@@ -43,6 +43,8 @@ struct DevAccel {
   int two_level;
   uint32_t* spill;          // single-level kernels: per-lane stack spill area, stack_cap entries
                             // per thread of the traversal grid (blockIdx.x * 256 + threadIdx.x)
+  uint32_t* touch_nodes;    // counting kernels only (else null): one bit per (bottom-level) node
+  uint32_t* touch_tris;     // and per triangle record, set when the launch reads it
 };
 
 // Queues are split into kSegs segments of `cap` slots each (slot = segment * cap + i).
@@ -62,13 +64,17 @@ constexpr uint32_t kLine = 32;  // uint32 words per 128-byte line
 //   [kCtrShadow + d] total shadow rays emitted at depth d; both written at the end of a
 //                    frame by k_sum_counts from the per-segment counters
 //   [kCtrOverflow]   bit 0: traversal stack overflow; bit 1: queue append failed
-//   [kCtrStats + k]  traversal statistics (node visits, triangle tests: 64-bit pairs)
+//   [kCtrStats + k]  traversal statistics (node visits, triangle tests, stack entries moved
+//                    from the LDS ring to the spill area: 64-bit pairs)
 //   [seg_ctr(kind, d) + s * kLine] per-segment counters, kinds below
 constexpr int kMaxDepth = 32;
 constexpr int kCtrExt = 0;
 constexpr int kCtrShadow = 32;
 constexpr int kCtrOverflow = 161;
-constexpr int kCtrStats = 164;  // 4 x uint64: closest nodes, closest tris, any nodes, any tris
+constexpr int kCtrStats = 164;  // 10 x uint64: closest nodes, closest tris, any nodes, any tris,
+                                // closest stack spills, any stack spills, then the distinct
+                                // nodes and triangle records read (summed over launches) by
+                                // closest and any traversals
 constexpr int kCtrSegBase = 256;
 enum SegKind : int { kSegExt = 0, kSegShadow = 1, kSegWorkExt = 2, kSegWorkShade = 3, kSegWorkShadow = 4 };
 constexpr int kNumSegKinds = 5;
@@ -135,6 +141,8 @@ void launch_shade(const SceneView& sv, const Material* sampling_mats, const Rend
 void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,
                    bool stats, hipStream_t stream);
 void launch_sum_counts(uint32_t* ctr, uint32_t cap, hipStream_t stream);
+// Adds the set bits of a touch bitmap (`words` 32-bit words) to *out and clears it.
+void launch_touch_count(uint32_t* bits, uint32_t words, unsigned long long* out, hipStream_t stream);
 void launch_tonemap(const float4* src, uint32_t n, const TonemapParams& tp, uchar4* dst, hipStream_t stream);
 void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
                      hipStream_t stream);

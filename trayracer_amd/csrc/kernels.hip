@@ -142,8 +142,13 @@ __device__ __forceinline__ uint32_t seg_append(uint32_t* cnt, uint32_t cap, uint
 struct TraceOut {
   float t, b1, b2;
   uint32_t prim, inst;
-  uint32_t nodes, tests;
+  uint32_t nodes, tests, spills;  // statistics: node visits, triangle tests, stack entries spilled
 };
+
+// Counting kernels: marks item i of a touch bitmap (null: not collected).
+__device__ __forceinline__ void touch(uint32_t* bits, uint32_t i) {
+  if (bits) atomicOr(&bits[i >> 5], 1u << (i & 31u));
+}
 
 // Ray/box slab test of the device traversals (Ize 2013's conservative form): each
 // distance is (plane - o) * inv, within 2^-23 relative of exact, and the far distance
@@ -419,10 +424,12 @@ __device__ __forceinline__ uint32_t* lane_spill(const TwoTierStack& s) {
   return s.spill_wave + lane_id() * static_cast<uint32_t>(s.cap);
 }
 
-__device__ __forceinline__ void stack_push(TwoTierStack& s, uint32_t v) {
+template <bool STATS>
+__device__ __forceinline__ void stack_push(TwoTierStack& s, uint32_t v, TraceOut& stats) {
   if (s.sp - s.lo == kRingEntries) {  // ring full: move its oldest entry out
     lane_spill(s)[s.lo] = s.ring[(s.lo & (kRingEntries - 1)) * 64];
     ++s.lo;
+    if (STATS) stats.spills++;
   }
   s.ring[(s.sp & (kRingEntries - 1)) * 64] = v;
   ++s.sp;
@@ -493,7 +500,8 @@ template <bool STATS, bool TWO>
 __device__ __forceinline__ bool leaf_phase(const LeafTasks& tk, const float4* __restrict__ tris, uint32_t lane,
                                            uint32_t fa, uint32_t ca, uint32_t fb, uint32_t cb, V3 o, int32_t kxyz,
                                            V3 shear, float tmin, float tmax, uint32_t prim0, uint32_t inst0,
-                                           uint32_t cur, bool& better, unsigned long long& kf, TraceOut& stats) {
+                                           uint32_t cur, bool& better, unsigned long long& kf, TraceOut& stats,
+                                           uint32_t* touch_tris) {
   const uint32_t n_tasks = ca + cb;  // at most 14
   uint32_t off = 0, total = 0;       // exclusive prefix and sum of n_tasks over the wave
   for (uint32_t bit = 0; bit < 4; ++bit) {
@@ -542,7 +550,10 @@ __device__ __forceinline__ bool leaf_phase(const LeafTasks& tk, const float4* __
     if (act) {
       const float4* tp = tris + static_cast<size_t>(ti) * 3u;
       const float4 a = tp[0], b = tp[1], cc = tp[2];
-      if (STATS) stats.tests++;
+      if (STATS) {
+        stats.tests++;
+        touch(touch_tris, ti);
+      }
       TriHit th;
       const uint32_t prim = __float_as_uint(a.w);
       if (intersect_triangle(ort, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), otmin, otmax, th) &&
@@ -626,7 +637,10 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
         float4 r0, r1, r2;
         int4 c;
         load_node(nodes, node, r0, r1, r2, c);
-        if (STATS) stats.nodes++;
+        if (STATS) {
+          stats.nodes++;
+          touch(acc.touch_nodes, static_cast<uint32_t>(node));
+        }
         float t0, t1;
         const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
         const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
@@ -637,7 +651,7 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
             farc = c.x;
           }
           if (st.sp < cap) {
-            stack_push(st, static_cast<uint32_t>(farc));
+            stack_push<STATS>(st, static_cast<uint32_t>(farc), stats);
           } else {
             atomicOr(overflow, 1u);
           }
@@ -665,7 +679,7 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
     unsigned long long kf;
     if (leaf_phase<STATS, false>(tk, tris, lane, ~static_cast<uint32_t>(leaf) >> 3, ca, ~static_cast<uint32_t>(node) >> 3,
                                  cb, rb.o, kxyz, shear, tmin, r.t, r.prim, kNoInstance, kNoInstance, better, kf,
-                                 stats) &&
+                                 stats, acc.touch_tris) &&
         ca + cb != 0) {
       if (better) {  // the barycentrics stay in tk.bary[lane] until the ray finishes
         r.t = key_t(kf);
@@ -735,7 +749,7 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
   bool ok = true;  // false after a stack overflow (reported once per lane)
   auto push = [&](uint32_t v) {
     if (st.sp < cap) {
-      stack_push(st, v);
+      stack_push<STATS>(st, v, stats);
     } else if (ok) {
       ok = false;
       atomicOr(overflow, 1u);
@@ -747,7 +761,7 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
       const int32_t e = stack_pop(st);
       if (e != kTravExit) return e;
       if (leaf != 0) {
-        stack_push(st, static_cast<uint32_t>(kTravExit));
+        stack_push<STATS>(st, static_cast<uint32_t>(kTravExit), stats);
         return kTravBlocked;
       }
       in_blas = false;
@@ -811,7 +825,10 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
         float4 r0, r1, r2;
         int4 c;
         load_node(in_blas ? blas : tlas, node, r0, r1, r2, c);
-        if (STATS) stats.nodes++;
+        if (STATS) {
+          stats.nodes++;
+          if (in_blas) touch(acc.touch_nodes, static_cast<uint32_t>(node));
+        }
         float t0, t1;
         const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
         const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
@@ -849,7 +866,8 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
     bool better;
     unsigned long long kf;
     if (leaf_phase<STATS, true>(tk, tris, lane, ~static_cast<uint32_t>(leaf) >> 3, ca, ~static_cast<uint32_t>(node) >> 3,
-                                cb, rb.o, kxyz, shear, tmin, r.t, r.prim, r.inst, cur_id, better, kf, stats) &&
+                                cb, rb.o, kxyz, shear, tmin, r.t, r.prim, r.inst, cur_id, better, kf, stats,
+                                acc.touch_tris) &&
         ca + cb != 0) {
       if (better) {  // the barycentrics stay in tk.bary[lane] until the ray finishes
         r.t = key_t(kf);
@@ -957,6 +975,17 @@ __global__ __launch_bounds__(64) void k_sum_counts(uint32_t* ctr, uint32_t cap) 
   ctr[(kind == kSegExt ? kCtrExt : kCtrShadow) + d] = total;
 }
 
+// Distinct items a launch read: set bits of its touch bitmap, which is cleared for the
+// next launch.
+__global__ __launch_bounds__(kBlock) void k_touch_count(uint32_t* bits, uint32_t words, unsigned long long* out) {
+  uint32_t n = 0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < words; i += gridDim.x * kBlock) {
+    n += static_cast<uint32_t>(__popc(bits[i]));
+    bits[i] = 0u;
+  }
+  wave_add_stat(out, n);
+}
+
 // Primary-ray slot of path p: segment p / per, in order (no atomics).
 __device__ __forceinline__ uint32_t primary_slot(uint32_t p, uint32_t per, uint32_t cap) {
   return (p / per) * cap + p % per;
@@ -1004,6 +1033,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
+  st.spills = 0;
   trace_queue<false, STATS>(
       acc, in, stk, tsk, &ctr[kCtrOverflow], st,
       [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
@@ -1020,6 +1050,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
   if (STATS) {
     wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 0]), st.nodes);
     wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 2]), st.tests);
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 8]), st.spills);
   }
 }
 
@@ -1032,6 +1063,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
+  st.spills = 0;
   trace_queue<true, STATS>(
       acc, in, stk, tsk, &ctr[kCtrOverflow], st,
       [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
@@ -1056,6 +1088,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
   if (STATS) {
     wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 4]), st.nodes);
     wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 6]), st.tests);
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 10]), st.spills);
   }
 }
 
@@ -1069,6 +1102,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
+  st.spills = 0;
   trace_queue2<false, STATS>(
       acc, in, stk, tsk, &ctr[kCtrOverflow], st,
       [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
@@ -1086,6 +1120,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
   if (STATS) {
     wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 0]), st.nodes);
     wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 2]), st.tests);
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 8]), st.spills);
   }
 }
 
@@ -1098,6 +1133,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
   TraceOut st;
   st.nodes = 0;
   st.tests = 0;
+  st.spills = 0;
   trace_queue2<true, STATS>(
       acc, in, stk, tsk, &ctr[kCtrOverflow], st,
       [&](uint32_t slot, V3& o, V3& d, float& tmin, float& tmax) {
@@ -1122,6 +1158,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
   if (STATS) {
     wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 4]), st.nodes);
     wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 6]), st.tests);
+    wave_add_stat(reinterpret_cast<unsigned long long*>(&ctr[kCtrStats + 10]), st.spills);
   }
 }
 
@@ -1549,6 +1586,12 @@ void launch_tonemap(const float4* src, uint32_t n, const TonemapParams& tp, ucha
 
 void launch_sum_counts(uint32_t* ctr, uint32_t cap, hipStream_t stream) {
   hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, stream, ctr, cap);
+}
+
+void launch_touch_count(uint32_t* bits, uint32_t words, unsigned long long* out, hipStream_t stream) {
+  const uint32_t blocks = (words + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_touch_count, dim3(blocks < 1024 ? (blocks ? blocks : 1) : 1024), dim3(kBlock), 0, stream, bits,
+                     words, out);
 }
 
 void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, float4* radiance,

@@ -100,10 +100,13 @@ TRT_FI S ggx_lambda(V3 w, const S& ax, const S& ay) {
   return (sqrt(1.f + a2t) - 1.f) * 0.5f;
 }
 
-// GTR1 (clearcoat) distribution, a < 1
-TRT_FI float gtr1_d(float cos_m, float a) {
+// GTR1 (clearcoat) distribution, a < 1, at the unit half vector m (local frame). The
+// denominator 1 + (a^2 - 1) cos^2 is formed as sin^2 + a^2 cos^2 from m's components:
+// near the peak the first form cancels to ~a^2 and loses most of its float precision
+// at glossy settings (a = 0.001).
+TRT_FI float gtr1_d(V3 m, float a) {
   const float a2 = a * a;
-  const float t = fma(a2 - 1.f, cos_m * cos_m, 1.f);
+  const float t = fma(a2, m.z * m.z, fma(m.x, m.x, m.y * m.y));
   return (a2 - 1.f) / (kPi * log(a2) * t);
 }
 
@@ -173,7 +176,7 @@ TRT_HD T3<S> bsdf_f(const BsdfParams<S>& p, V3 wo, V3 wi) {
   // clearcoat
   if (p.clearcoat > 0.f) {
     const float a = lerp(0.1f, 0.001f, p.clearcoat_gloss);
-    const float Dr = gtr1_d(h.z, a);
+    const float Dr = gtr1_d(h, a);
     const float Fr = lerp(0.04f, 1.f, FH);
     const float Gr = smith_g1_ggx_iso(cl, 0.25f) * smith_g1_ggx_iso(cv, 0.25f);
     const float kc = 0.25f * p.clearcoat * Dr * Fr * Gr / (4.f * cl * cv);
@@ -214,7 +217,7 @@ TRT_HD inline float bsdf_pdf(const BsdfParams<float>& p, V3 wo, V3 wi) {
     pdf += ps * (g1 * ggx_d(h, ax, ay) / (4.f * wo.z));
     if (pc > 0.f) {
       const float a = lerp(0.1f, 0.001f, p.clearcoat_gloss);
-      pdf += pc * (gtr1_d(h.z, a) * h.z / (4.f * oh));
+      pdf += pc * (gtr1_d(h, a) * h.z / (4.f * oh));
     }
   }
   return pdf;

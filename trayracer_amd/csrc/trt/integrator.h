@@ -38,7 +38,11 @@ struct alignas(16) RenderParams {
   float zdepth_max; // depth of white in the z-depth mode (KernelSettings::zDepthMax)
   uint32_t mode;    // render mode (render_modes.h); 0 = path tracing
   uint32_t mode_flags;  // material property shown by the material-property mode
-  uint32_t pad0, pad1;
+  uint32_t separate_sampling;  // 1: the sampling materials are a set of their own
+                               // (differentiable renders), see the Russian roulette
+  uint32_t strategy;  // emitter estimators: 0 = both, MIS-weighted (the renderer always);
+                      // 1 = light sampling only, 2 = BSDF sampling only (each alone is
+                      // unbiased too: the oracle's tests compare the three means)
 };
 
 struct PathState {
@@ -158,9 +162,9 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
     if (cos_l > 0.f) {
       if (depth == 0 || (ps.flags & kFlagSpecular)) {
         out.L = clamp_contrib(ps.T * le, rp.clamp);
-      } else {
+      } else if (rp.strategy != 1u) {
         const float pl = light_pdf_hit(s, le, sp.area, cos_l, h.t);
-        const float w = ps.pdf / (ps.pdf + pl);
+        const float w = rp.strategy == 2u ? 1.f : ps.pdf / (ps.pdf + pl);
         out.L = clamp_contrib(ps.T * le * w, rp.clamp);
       }
     }
@@ -192,7 +196,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
   }
 
   // next-event estimation
-  if (nee) {
+  if (nee && rp.strategy != 2u) {
     float b1, b2;
     sample_triangle(u[2], u[3], b1, b2);
     const V3 p0 = v3(L.p0[0], L.p0[1], L.p0[2]);
@@ -212,7 +216,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
           const T3<float> f = bsdf_f(bp, wo, wi);
           const float pl = (L.power / s.total_power) * dist2 / (L.area * cos_l);
           const float pb = bsdf_pdf(bs, wo, wi);
-          const float w = pl / (pl + pb);
+          const float w = rp.strategy == 1u ? 1.f : pl / (pl + pb);
           const float k = wi.z * (w / pl);
           const V3 c = v3(ps.T.x * f.x * L.le[0] * k, ps.T.y * f.y * L.le[1] * k, ps.T.z * f.z * L.le[2] * k);
           if (!is_black(c)) {
@@ -245,7 +249,19 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
         V3 Tn = v3(ps.T.x * f.x * k, ps.T.y * f.y * k, ps.T.z * f.z * k);
         bool alive = !is_black(Tn);
         if (alive && depth >= rp.rr_depth) {
-          const float q = fmin(1.f, max3(Tn));
+          // Russian roulette (PathTracing.h:148-154): survival probability
+          // min(1, max throughput). With a set of sampling materials of their own
+          // (differentiable renders) it is a sampling decision like the others: this
+          // vertex's BSDF weight under the sampling materials alone (the throughput
+          // carries earlier vertices' evaluation parameters), which the adjoint holds
+          // fixed, so the continuation's derivative stays (df/dtheta) / f
+          float q;
+          if (rp.separate_sampling) {
+            const T3<float> fs = bsdf_f(bs, wo, smp.wi);
+            q = fmin(1.f, fmax(fmax(fs.x, fs.y), fs.z) * k);
+          } else {
+            q = fmin(1.f, max3(Tn));
+          }
           if (u[7] >= q) alive = false;
           else Tn = Tn * (1.f / q);
         }

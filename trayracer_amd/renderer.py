@@ -205,6 +205,10 @@ class Renderer:
     def SampleCount(self):
         return int(self.lib.trt_sample_count(self.ctx))
 
+    def frame_id(self):
+        """Frames rendered by this context so far (changes with every RenderFrame)."""
+        return int(self.lib.trt_frame_id(self.ctx))
+
     def Statistics(self):
         return self.last_stats
 

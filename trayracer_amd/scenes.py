@@ -115,14 +115,16 @@ def material_params(materials):
     return np.concatenate([m[:, 0:3], m[:, 3:4], m[:, 7:8]], axis=1).copy()
 
 
-def cornell_diff(width=512, height=512, spp=64, max_depth=4, seed=3, sigma=0.1):
+def cornell_diff(width=512, height=512, spp=64, max_depth=4, seed=3, sigma=0.1, rr_depth=None):
     """Differentiable Cornell box: the Cornell geometry with one Disney material per
     triangle (36 x 5 = 180 parameters: base colour rgb, roughness, metallic). Returns
     (scene, target_params): the target is the scene's parameters perturbed by
-    N(0, sigma) and clipped to their valid ranges. Russian roulette starts at the last
-    vertex (rr_depth = max_depth - 1), where no extension is traced, so the estimator is
-    a smooth function of the parameters."""
-    sc = cornell(width=width, height=height, spp=spp, max_depth=max_depth, rr_depth=max_depth - 1, seed=seed)
+    N(0, sigma) and clipped to their valid ranges. Russian roulette starts at rr_depth
+    (default: the last vertex, max_depth - 1, where no extension is traced); its survival
+    probability comes from the sampling parameters, so with those held fixed the
+    estimator is a smooth function of the evaluation parameters either way."""
+    rr = max_depth - 1 if rr_depth is None else rr_depth
+    sc = cornell(width=width, height=height, spp=spp, max_depth=max_depth, rr_depth=rr, seed=seed)
     base = sc["materials"][sc["mat_ids"]].copy()
     rng = np.random.default_rng(seed)
     for i, m in enumerate(base):
