@@ -6,9 +6,10 @@ queues) and the large-tile convolution kernels run:
 
 * 1M-triangle soup, pre-split budget 6 (6.7M references, depth ~27), 1920x1080, 8 spp,
   depth 8: closest/any hits of random and camera rays bit-exact against the oracle's
-  independent BVH; the bench frame rendered in full and a 256x256 window of its paths
-  compared with the oracle (same keys) to 1e-4; the traversal stack never overflows and
-  its spill area is used.
+  independent BVH; the bench frame rendered in full, its first-hit AOVs over the whole
+  film equal to the oracle's, and (under the sky, since the opaque soup renders black) a
+  256x256 window of the frame's paths compared with the oracle (same keys) to 1e-4; the
+  traversal stack never overflows and its spill area is used.
 * 1000 instances of a 20,480-triangle icosphere with 10,000 emitters, 3840x2160, 4 spp,
   depth 6: the same ray and window checks.
 * The U-Net at 1920x1080 on features of 1-spp renders of both scenes (trt_denoise, the
@@ -104,10 +105,18 @@ def test_config1_rays_exact(rend, soup_full):
     assert 0.05 < occ_g.mean() < 0.95
 
 
-def test_config1_bench_frame_paths(rend, soup_full):
+SKY = dict(enabled=True, draw_sun=True, dir=(0.3, 0.8, -0.5), angular_diameter=32.0, intensity=1000.0,
+           turbidity=4.0)
+
+
+def test_config1_bench_frame(rend, soup_full):
     """The benchmark's frame (1920x1080, 8 spp, depth 8, RR from depth 3) rendered in
-    full on the GPU; the paths of a 256x256 window at the film's centre against the
-    oracle; the counting pass over the same frame reports the traversal stack's spills."""
+    full on the GPU. The soup is opaque (1M triangles in the unit cube: a mean free path
+    of ~0.01) and its 64 emitters lie inside it, so no light reaches the camera and every
+    path's radiance is zero, on the GPU and in the oracle alike; the frame is checked
+    through its first hits instead: the albedo, normal and view-depth AOVs of all 2M
+    pixels against the oracle's (sample 0). The counting pass over the same frame reports
+    the traversal stack's spills."""
     sc, orc = soup_full
     rend.UpdateScene(sc, presplit=6.0)
     st = rend.RenderFrame()
@@ -115,15 +124,36 @@ def test_config1_bench_frame_paths(rend, soup_full):
     W, H, spp = r["width"], r["height"], r["spp"]
     assert st["overflow"] == 0 and st["paths"] == W * H * spp
     print("config-1 frame:", {k: st[k] for k in ("extend_rays", "shadow_rays", "frame_ms")})
-    pg = rend.path_radiance()
-    crop = (832, 412, 256, 256)
-    ref = orc.render(crop=crop)
-    exact = _compare_paths(_window(pg, spp, W, H, crop), ref["path_rgb"], "1M soup 1080p window")
-    print("bit-exact per-path radiance:", exact)
+    assert not rend.path_radiance().any()
+    ref = orc.render(render=dict(spp=1, max_depth=1))  # sample 0's first hits
+    np.testing.assert_array_equal(rend.albedo()[..., :3], ref["aov"][..., 0:3])
+    np.testing.assert_array_equal(rend.normal(), ref["aov"][..., 4:8])
+    assert (ref["aov"][..., 0:3] > 0).any(axis=2).mean() > 0.5
+    rend.Reset()  # the counting frame then traces the same paths (sample offset 0)
     tc = rend.measure_traversal()
     print("config-1 traversal counts:", tc)
     assert tc["closest_rays"] == st["extend_rays"] and tc["any_rays"] == st["shadow_rays"]
     assert tc["closest_spills"] > 0, "the 1M soup never used the traversal stack's spill area"
+
+
+def test_config1_sky_lit_window_paths(rend, soup_full):
+    """The same frame under the daylight sky (the reference's default sky is on): paths
+    that leave the soup carry sky radiance, so a 256x256 window at the film's centre (8
+    spp, the GPU frame rendered in full) compares non-zero per-path radiance."""
+    from oracle import Oracle
+    sc, _ = soup_full
+    sc = dict(sc, sky=SKY)
+    rend.UpdateScene(sc, presplit=6.0)
+    st = rend.RenderFrame()
+    r = sc["render"]
+    W, H, spp = r["width"], r["height"], r["spp"]
+    assert st["overflow"] == 0
+    pg = rend.path_radiance()
+    crop = (832, 412, 256, 256)
+    ref = Oracle(sc).render(crop=crop)
+    assert (ref["path_rgb"] > 0).any(axis=1).mean() > 0.3
+    exact = _compare_paths(_window(pg, spp, W, H, crop), ref["path_rgb"], "1M soup 1080p window, sky")
+    print("bit-exact per-path radiance:", exact)
 
 
 # ---------------------------------------------------------------------------------------
@@ -263,7 +293,10 @@ def test_config4_denoise_1080p_on_renders(rend, which):
     persistent 16-row-tile convolutions run at the full and half resolution levels)."""
     from oracle import unet_ref as U
     W, H = 1920, 1080
-    sc = scenes.soup(width=W, height=H, spp=1) if which == "soup" else scenes.instanced(width=W, height=H, spp=1)
+    if which == "soup":  # sky-lit: the soup alone renders black (see test_config1_bench_frame)
+        sc = dict(scenes.soup(width=W, height=H, spp=1), sky=SKY)
+    else:
+        sc = scenes.instanced(width=W, height=H, spp=1)
     rend.UpdateScene(sc)
     rend.RenderFrame()
     params = U.init_params(21 if which == "soup" else 22)
