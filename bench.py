@@ -172,36 +172,39 @@ def network_roofline(rend, calls=10):
 
 
 def denoise_metric(rend, steps):
-    """Time to relMSE 0.01 against a 4096-spp render, plain accumulation vs learned
-    denoising, on the Cornell box at 1920x1080 seen from its own (held-out) camera. The
-    denoiser is first trained on the GPU from frames the tracer renders (training time
-    reported separately); the instanced scene's view is reported beside it."""
-    import math
-
+    """Time to relMSE 0.01 against a converged render, plain accumulation vs learned
+    denoising, on the two scenes the denoiser's inputs come from: the 1M-triangle soup
+    (lit by the sky: alone it renders black) and the 1000-instance scene, at 1920x1080,
+    each from a camera that is not among the training views. The denoiser is first
+    trained on the GPU from frames the tracer renders of both scenes under random
+    cameras (training time reported separately)."""
     from trayracer_amd import convergence as C
-    from trayracer_amd import scenes
 
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     rend.denoiser_init(seed=0)
-    t_pairs = C.build_training_set(rend, views=(("cornell", 6), ("inst", 3), ("soup", 1)), log=log)
+    t_pairs = C.build_training_set(rend, views=(("soup", 4), ("inst", 4)), log=log)
     t_train, losses = C.train(rend, steps=steps, batch=8, crop=128, lr=1e-3)
     log(f"denoiser trained: {steps} steps in {t_train:.1f} s, loss {float(losses[-50:].mean()):.5f}")
-    sc = scenes.cornell(width=1920, height=1080)
-    res = C.time_to_relmse(rend, sc, sc["camera"], max_plain_spp=4096, log=log)
-    inst = C.time_to_relmse(rend, scenes.instanced(), C.INST_CAMERA, max_plain_spp=256, max_denoise_spp=16, log=log)
+    soup = C.time_to_relmse(rend, C.make_scene("soup"), C.SOUP_CAMERA, max_plain_spp=2048, max_denoise_spp=64, log=log)
+    inst = C.time_to_relmse(rend, C.make_scene("inst"), C.INST_CAMERA, max_plain_spp=256, max_denoise_spp=64, log=log)
 
-    def pick(r):
-        out = {k: r[k] for k in ("target", "plain_time_s", "plain_spp", "plain_how", "denoised_time_s",
-                                 "denoised_spp", "speedup", "reference_spp")}
+    def pick(r, view):
+        out = {"view": view}
+        out.update({k: r[k] for k in ("target", "plain_time_s", "plain_spp", "plain_how", "denoised_time_s",
+                                      "denoised_spp", "speedup", "reference_spp", "failure")})
+        out["reference_relmse"] = round(r["reference_relmse"], 5)
         out["plain_relmse_by_spp"] = {int(k): round(e, 5) for k, _, e in r["plain_curve"]}
         out["denoised_relmse_by_spp"] = {int(k): round(e, 5) for k, _, e, _ in r["denoised_curve"]}
         out["denoise_ms"] = round(r["denoised_curve"][-1][3], 3)
         return out
-    return {"view": "cornell 1920x1080, its own camera (not a training view), depth 4",
-            "relmse": "mean (x - r)^2 / (r^2 + 1e-2) vs a 4096-spp render", **pick(res),
+    return {"relmse": "mean (x - r)^2 / (r^2 + 1e-2) vs the mean of two independent reference halves; "
+                      "targets compare relMSE minus the reference's estimated own error",
+            "soup": pick(soup, "1M-triangle soup under the sky, 1920x1080, camera (0.5, 0.5, -1.2), depth 8"),
+            "instanced": pick(inst, "1000-instance scene, 1920x1080, camera (50, 50, -20), depth 6"),
             "train": {"pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1), "steps": steps, "batch": 8,
-                      "crop": 128, "final_loss": float(losses[-50:].mean())},
-            "instanced_view": pick(inst), "network_roofline": network_roofline(rend)}
+                      "crop": 128, "views": "4 soup (sky) + 4 instanced random cameras, inputs 1/2/4/8 spp, "
+                                            "targets 1024 spp", "final_loss": float(losses[-50:].mean())},
+            "network_roofline": network_roofline(rend)}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_extend_sl_soup1m_traffic.json")
@@ -221,13 +224,15 @@ def traversal_source_hash():
 
 def gather_ceilings():
     """Rates of independent 48-byte random gathers (trt_measure_gather) from a buffer that
-    fits the L2 (4 MiB), one that fits the Infinity Cache (128 MiB) and one in HBM (4 GiB)."""
+    fits a CU's L1 (16 KiB), one that fits the L2 (4 MiB), one that fits the Infinity
+    Cache (128 MiB) and one in HBM (4 GiB)."""
     import ctypes
 
     from trayracer_amd import _lib
     lib = _lib.load()
     out = {}
-    for name, size in (("l2_4MiB", 4 << 20), ("mall_128MiB", 128 << 20), ("hbm_4GiB", 4 << 30)):
+    for name, size in (("l1_16KiB", 16 << 10), ("l2_4MiB", 4 << 20), ("mall_128MiB", 128 << 20),
+                       ("hbm_4GiB", 4 << 30)):
         g = ctypes.c_double()
         st = lib.trt_measure_gather(0, size, 256, ctypes.byref(g))
         out[name] = round(g.value, 1) if st == 0 else None
@@ -244,7 +249,8 @@ def extend_roofline(tc, info, ext_rays, ext_ms, launches, workload):
       null when the profile was taken from other kernel sources;
     * vector memory: the bytes every ray loads (ray and hit records, node visits x 48 B,
       triangle tests x 48 B) against the rate of independent random 48-byte gathers from
-      an L2-resident buffer (trt_measure_gather), the fastest such loads go on this GPU."""
+      an L1-resident buffer (trt_measure_gather), the fastest such loads go through the
+      CUs' load path (Extend's node loads mostly hit L1 and L2)."""
     n = max(1, launches)
     avg_s = ext_ms / n * 1e-3
     per_ray = 48 + tc["closest_nodes_per_ray"] * tc["node_bytes"] + tc["closest_tris_per_ray"] * tc["tri_bytes"]
@@ -262,7 +268,7 @@ def extend_roofline(tc, info, ext_rays, ext_ms, launches, workload):
             traffic = round(prof["traffic_bytes_per_launch"])
     ceil = gather_ceilings()
     loaded = per_ray * rays_per_launch / avg_s / 1e9
-    vm_peak = ceil["l2_4MiB"]
+    vm_peak = ceil["l1_16KiB"]
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -279,27 +285,33 @@ def extend_roofline(tc, info, ext_rays, ext_ms, launches, workload):
                           "frac": round(loaded / vm_peak, 4) if vm_peak else None,
                           "bytes_per_ray": round(per_ray, 1), "nodes_per_ray": round(tc["closest_nodes_per_ray"], 2),
                           "tris_per_ray": round(tc["closest_tris_per_ray"], 2),
-                          "peak_source": "trt_measure_gather: independent random 48-byte gathers, 4 MiB buffer",
+                          "peak_source": "trt_measure_gather: independent random 48-byte gathers, 16 KiB buffer",
                           "gather_ceilings_gbs": ceil},
         "stack_spills_per_ray": round(tc["closest_spills"] / max(1, tc["closest_rays"]), 4),
     }
 
 
 def shade_roofline(scene, stats, steps, info):
-    """Shade's HBM roofline: algorithmic bytes per shaded vertex (the surface's records and
-    the path's queue traffic) over the frame's Shade time. Per live path: queue record
-    in (52 B: origin, direction, throughput, path) + hit (16 B, + 4 B instance) + triangle
-    positions (36 B, + 36 B normals) + material id (4 B) + material (80 B) + light record
-    and alias entry (88 B) + radiance read-modify-write (32 B) + extension record out (52 B)
-    + shadow record out (48 B)."""
+    """Shade's rooflines per frame. HBM: the bytes a path's vertex must stream at least,
+    its queue record in (52 B: origin, direction, throughput, path), its hit (16 B, + 4 B
+    instance), the radiance read-modify-write (32 B), and the records it writes, an
+    extension (52 B) for each next-bounce ray and a shadow ray (48 B) for each shadow ray
+    emitted, over the frame's Shade time against 8 TB/s. The surface data it gathers
+    (triangle 36 B, + 36 B normals, material id 4 B, material 80 B, light record and alias
+    entry 88 B per shaded hit) is mostly served by the caches and is reported beside."""
     two = bool(info["two_level"])
     nrm = scene.get("normals") is not None
-    per = 52 + 16 + (4 if two else 0) + 36 + (36 if nrm else 0) + 4 + 80 + 88 + 32 + 52 + 48
     ext = sum(s["extend_rays"] for s in stats) / steps
+    prim = sum(s["primary_rays"] for s in stats) / steps
+    shadow = sum(s["shadow_rays"] for s in stats) / steps
     ms = sum(s["shade_ms"] for s in stats) / steps
-    achieved = per * ext / (ms * 1e-3) / 1e9
+    stream = ext * (52 + 16 + (4 if two else 0) + 32) + (ext - prim) * 52 + shadow * 48
+    gathered = 36 + (36 if nrm else 0) + 4 + 80 + 88
+    achieved = stream / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_path": per, "paths_per_frame": round(ext),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "streamed_bytes_per_frame": round(stream),
+            "gathered_bytes_per_path": gathered,
+            "gathered_gbs": round(gathered * ext / (ms * 1e-3) / 1e9, 1), "paths_per_frame": round(ext),
             "shade_ms_per_frame": round(ms, 3),
             "valu_profile": "profiles/wavefront_soup1m_counters.md (VALU load, lane use, TA/TD busy per kernel)"}
 

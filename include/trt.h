@@ -94,7 +94,7 @@ typedef struct trt_render_opts {
   float ray_epsilon;
   float clamp;         /* firefly clamp on a contribution's max component; 0 = off */
   uint32_t mode;       /* render mode (replaces Renderer::SetRenderMode, Tracer/Renderer/Renderer.h:48-49,
-                          with the kernels of Tracer/CUDA/GPU/Shade/*.h): TRT_MODE_* below */
+                          with the kernels of the Tracer/CUDA/GPU/Shade headers): TRT_MODE_ values below */
   uint32_t mode_flags; /* material property for TRT_MODE_MATERIAL_PROPERTY (the reference's
                           MaterialPropertyIds order: anisotropic 0, clearcoat, clearcoat gloss, diffuse,
                           emissive, metallic, normal, roughness, sheen, sheen tint, specular,
@@ -261,9 +261,14 @@ trt_status trt_write_png(const char* path, uint32_t width, uint32_t height, cons
 
 /* Sky seen by rays that leave the scene (replaces Sky and SampleSky,
  * Tracer/Renderer/Sky.h:10-66, Tracer/CUDA/GPU/CudaSky.h:44-156): the reference's
- * Preetham-style daylight model. Not sampled by next-event estimation (the reference
- * compiles sky NEE out, Tracer/CUDA/GPU/CudaLights.h:11); the sun disc is drawn for camera
- * rays only. Resets accumulation. Disabled by default (trt_clear_scene disables it). */
+ * Preetham-style daylight model. With sun_nee = 0 (the reference's setting: it compiles
+ * sky NEE out, Tracer/CUDA/GPU/CudaLights.h:11) the sky is reached only by rays that
+ * miss and the sun disc is drawn for camera rays and specular bounces only. With
+ * sun_nee = 1 the sun disc is also a light of next-event estimation, chosen against
+ * the emissive triangles by energy (the reference's disabled branch,
+ * Tracer/CUDA/GPU/CudaLights.h:24, 66-88), and rays that reach the disc add it with the
+ * balance-heuristic MIS weight. Resets accumulation. Disabled by default
+ * (trt_clear_scene disables it). */
 typedef struct trt_sky {
   uint32_t enabled;
   uint32_t draw_sun;
@@ -271,6 +276,7 @@ typedef struct trt_sky {
   float angular_diameter;    /* sun disc, arc minutes (32) */
   float intensity;           /* sun intensity (1000) */
   float turbidity;           /* atmosphere turbidity (4) */
+  uint32_t sun_nee;          /* 1: the sun disc is a light of next-event estimation */
 } trt_sky;
 trt_status trt_set_sky(trt_context* ctx, const trt_sky* sky);
 
@@ -293,6 +299,14 @@ trt_status trt_read_accum(trt_context* ctx, float* rgba);
 trt_status trt_read_color(trt_context* ctx, float* rgba);
 trt_status trt_read_albedo(trt_context* ctx, float* rgba);
 trt_status trt_read_normal(trt_context* ctx, float* rgba);
+/* Checkpoint / resume of progressive rendering (no counterpart in the reference, which
+ * restarts accumulation on every reset, Tracer/Renderer/Renderer.cpp:186-190): restores the
+ * accumulator of a width x height film from rgba as trt_read_accum wrote it (rgb sums, the
+ * sample count in alpha, equal in every pixel); the next trt_render of that size
+ * continues with the following samples, so the resumed render equals the uninterrupted
+ * one. The scene and camera must be set as they were (setting them restarts
+ * accumulation). */
+trt_status trt_write_accum(trt_context* ctx, uint32_t width, uint32_t height, const float* rgba);
 /* Per-path radiance of the last frame, 3 floats per path (path = sample*W*H + pixel). */
 trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths);
 
@@ -388,6 +402,17 @@ trt_status trt_denoiser_train(trt_context* ctx, uint32_t steps, uint32_t batch, 
                               float* losses);
 /* Denoises the accumulated frame (needs width % 128 == 0, height % 8 == 0). */
 trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* stats);
+/* The reference's denoising policy (Denoiser::SetEnabled / SetSampleTreshold,
+ * Tracer/Optix/Denoiser.h:24-31; Renderer::ShouldDenoise and DenoiseFrame,
+ * Tracer/Renderer/Renderer.cpp:291-304, 546-569): trt_denoise_frame, called after a frame,
+ * denoises when the policy is enabled, the last frame was path traced (or ambient
+ * occlusion), at least `sample_threshold` samples are accumulated and, after a first
+ * denoise, the sample count reached 1.5 x the one last denoised; the result is blended
+ * towards the noisy frame by 1 / (spp + 1) (Tracer/Optix/Denoiser.cpp:108). *ran = 1 when
+ * it denoised (stats then filled). Restarting accumulation restarts the policy. Off by
+ * default. */
+trt_status trt_denoiser_set_policy(trt_context* ctx, uint32_t enabled, uint32_t sample_threshold);
+trt_status trt_denoise_frame(trt_context* ctx, trt_denoise_stats* stats, uint32_t* ran);
 trt_status trt_read_denoised(trt_context* ctx, float* rgba);
 /* Test hooks: network forward on host features [N][H][W][16] -> log-space output
  * [N][H][W][32]; backward of sum(dout * output) -> parameter gradients. */

@@ -147,6 +147,7 @@ struct OScene {
   SceneView view;
   std::vector<Material> samp;  // sampling materials (differentiable renders)
   bool separate_sampling = false;  // set by orc_set_material_params, as trt_set_material_params
+  float radius = 0.f;              // half-diagonal of the world bounds (the sun's selection weight)
   // CPU-baseline mode only (orc_scene_use_product_bvh; never used by the parity checks):
   // the product's binned-SAH tree over pre-split references, traversed near child first
   bool use_product_bvh = false;
@@ -425,6 +426,8 @@ void* orc_scene_create(const float* pos, const float* nrm, const uint32_t* mat_i
   }
   s->lt = gather_lights(s->pos.data(), s->mat_ids.data(), n_tris, s->mats.data(), n_mats,
                         n_inst ? s->inst.data() : nullptr, n_inst, s->mesh_cnt.data());
+  s->radius = scene_radius(s->pos.data(), n_tris, n_inst ? s->inst.data() : nullptr, n_inst, s->mesh_off.data(),
+                           s->mesh_cnt.data(), uint32_t(s->mesh_off.size()));
   SceneView& v = s->view;
   std::memset(&v, 0, sizeof(v));
   v.tri_pos = s->pos.data();
@@ -494,9 +497,10 @@ void orc_set_material_params(void* p, const float* params, uint32_t n, uint32_t 
 
 // Sky seen by rays that leave the scene (same parameters as trt_set_sky).
 void orc_set_sky(void* p, uint32_t enabled, uint32_t draw_sun, const float* dir, float angular_diameter,
-                 float intensity, float turbidity) {
+                 float intensity, float turbidity, uint32_t sun_nee) {
   OScene* s = static_cast<OScene*>(p);
-  s->view.sky = make_sky(enabled != 0, draw_sun != 0, dir, angular_diameter, intensity, turbidity);
+  s->view.sky = make_sky(enabled != 0, draw_sun != 0, dir, angular_diameter, intensity, turbidity, sun_nee != 0);
+  s->view.sky.sun_power = sun_selection_power(s->view.sky, s->radius);
 }
 
 // Sky radiance along n unit directions (3 floats each) -> out (3 floats each).
@@ -528,6 +532,7 @@ void orc_tone_curve(const float* x, uint32_t n, uint32_t method, float* out) {
 void orc_scene_destroy(void* p) { delete static_cast<OScene*>(p); }
 
 uint32_t orc_light_count(void* p) { return uint32_t(static_cast<OScene*>(p)->lt.lights.size()); }
+float orc_sun_power(void* p) { return static_cast<OScene*>(p)->view.sky.sun_nee ? static_cast<OScene*>(p)->view.sky.sun_power : 0.f; }
 float orc_total_power(void* p) { return static_cast<OScene*>(p)->lt.total_power; }
 // lights: 20 floats per light (LightTri layout); prob/alias per light
 void orc_light_table(void* p, float* lights, float* prob, uint32_t* alias) {

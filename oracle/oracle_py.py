@@ -20,6 +20,7 @@ _PROTO = [
     ("orc_scene_destroy", None, [_P]),
     ("orc_scene_use_product_bvh", ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int]),
     ("orc_light_count", ctypes.c_uint32, [_P]),
+    ("orc_sun_power", ctypes.c_float, [_P]),
     ("orc_total_power", ctypes.c_float, [_P]),
     ("orc_light_table", None, [_P, _f, _f, _u]),
     ("orc_trace_closest", None, [_P, _f, ctypes.c_uint32, _f, _u, ctypes.c_int, ctypes.c_int]),
@@ -29,7 +30,8 @@ _PROTO = [
     ("orc_tonemap", None, [_f, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                            ctypes.POINTER(ctypes.c_uint8)]),
     ("orc_tone_curve", None, [_f, ctypes.c_uint32, ctypes.c_uint32, _f]),
-    ("orc_set_sky", None, [_P, ctypes.c_uint32, ctypes.c_uint32, _f, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("orc_set_sky", None, [_P, ctypes.c_uint32, ctypes.c_uint32, _f, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                           ctypes.c_uint32]),
     ("orc_sky_radiance", None, [_P, _f, ctypes.c_uint32, ctypes.c_uint32, _f]),
     ("orc_backward", None, [_P, _f, _u, _f, _f, _d, ctypes.c_int]),
     ("orc_pcg32", None, [ctypes.c_uint64, ctypes.c_uint64, _u, ctypes.c_uint32]),
@@ -116,11 +118,12 @@ class Oracle:
             self.set_sky(scene["sky"])
 
     def set_sky(self, sky):
-        """sky: dict(enabled, draw_sun, dir, angular_diameter (arc minutes), intensity, turbidity)."""
+        """sky: dict(enabled, draw_sun, dir, angular_diameter (arc minutes), intensity, turbidity,
+        sun_nee)."""
         d = np.ascontiguousarray(sky.get("dir", (0.0, 1.0, 0.0)), np.float32)
         self.lib.orc_set_sky(self.handle, int(sky.get("enabled", True)), int(sky.get("draw_sun", True)), fp(d),
                              float(sky.get("angular_diameter", 32.0)), float(sky.get("intensity", 1000.0)),
-                             float(sky.get("turbidity", 4.0)))
+                             float(sky.get("turbidity", 4.0)), int(sky.get("sun_nee", False)))
 
     def sky_radiance(self, dirs, sun=True):
         dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
@@ -142,6 +145,10 @@ class Oracle:
         out = np.empty((rays.shape[0], 11), np.float32)
         self.lib.orc_surface(self.handle, fp(rays), rays.shape[0], fp(out))
         return dict(p=out[:, 0:3], ng=out[:, 3:6], ns=out[:, 6:9], area=out[:, 9], mat=out[:, 10].view(np.uint32))
+
+    def sun_power(self):
+        """Selection weight of the sun disc in next-event estimation (0 unless sun_nee)."""
+        return float(self.lib.orc_sun_power(self.handle))
 
     def close(self):
         if self.handle:

@@ -282,7 +282,11 @@ def _check_network(rend, feats, params, N, H, W, rad=None, full=False):
             want = np.expm1(np.maximum(z, 0.0))
             got = rad[:, r0:r1, :, :3].astype(np.float64)
             d = np.abs(got - want) / (1.0 + want)  # error of log(1 + radiance)
-            assert float(d.max()) < 2e-4 * max(1.0, float(np.abs(z).max())), float(d.max())
+            # the device rounds its own log1p of the radiance to bf16: that feature may be
+            # one bf16 ulp from the host's (same rounding of a value within an ulp of a tie)
+            f = np.abs(U.bf16(feats[:, r0:r1, :, :3]))
+            ulp = np.where(f > 0, 2.0 ** (np.floor(np.log2(np.maximum(f, 1e-30))) - 7), 0.0)
+            assert (d < 2e-4 * max(1.0, float(np.abs(z).max())) + ulp).all(), float(d.max())
             assert (rad[:, r0:r1, :, 3] == 1.0).all()
     return acts
 

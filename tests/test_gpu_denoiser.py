@@ -160,3 +160,43 @@ def test_training_is_deterministic(rend):
     l2 = rend.denoiser_train(5, batch=2, crop=128, lr=1e-3, seed=11)
     assert np.array_equal(l1, l2)
     assert np.array_equal(rend.denoiser_params(), w1)
+
+
+def test_denoising_policy_and_blend(rend):
+    """The reference's policy (Renderer::ShouldDenoise / DenoiseFrame, Renderer.cpp:291-304,
+    546-569): nothing below the sample threshold or when disabled; then a denoise at the
+    threshold and again only once the sample count reaches 1.5x the last denoised one; a
+    restart of accumulation restarts it. The result is the network's output blended
+    towards the noisy frame by 1 / (spp + 1) (Denoiser.cpp:108)."""
+    W, H = 256, 128
+    sc = scenes.cornell(width=W, height=H, spp=1, max_depth=4, seed=3)
+    rend.UpdateScene(sc)
+    rend.denoiser_init(seed=2)
+    rend.SetDenoiser(False, 4)
+    rend.RenderFrame()
+    assert rend.DenoiseFrame() is None  # disabled
+    rend.SetDenoiser(True, 4)
+    ran = []
+    for _ in range(14):
+        rend.RenderFrame()
+        if rend.DenoiseFrame() is not None:
+            ran.append(rend.SampleCount())
+    assert ran == [4, 6, 9, 13], ran
+    # blend: against the unblended network output of the same frame
+    blended = rend.denoised()[..., :3].astype(np.float64)
+    spp = rend.SampleCount()
+    rend.denoise()  # plain trt_denoise: no blend
+    raw = rend.denoised()[..., :3].astype(np.float64)
+    col = rend.color()[..., :3].astype(np.float64)
+    b = 1.0 / (spp + 1)
+    np.testing.assert_allclose(blended, (1 - b) * raw + b * col, rtol=1e-5, atol=1e-6)
+    rend.Reset()
+    rend.RenderFrame()
+    assert rend.DenoiseFrame() is None  # 1 sample after the restart
+    for _ in range(3):
+        rend.RenderFrame()
+    assert rend.DenoiseFrame() is not None  # threshold reached again
+    rend.SetRenderMode(4)  # geometric normals: never denoised (ShouldDenoise)
+    for _ in range(5):
+        rend.RenderFrame()
+    assert rend.DenoiseFrame() is None

@@ -330,13 +330,16 @@ def test_pick_ray(rend):
     assert abs(p["direction"][2] - 1.0) < 0.01
 
 
-def test_path_radiance_with_sky(rend):
+@pytest.mark.parametrize("sun_nee", [False, True], ids=["sky", "sky+sun-nee"])
+def test_path_radiance_with_sky(rend, sun_nee):
     """Open Cornell box under the daylight sky: rays leaving through the open side and
-    camera rays that miss pick up sky radiance (sun disc for camera rays)."""
+    camera rays that miss pick up sky radiance (sun disc for camera rays); with sun_nee
+    the sun disc is also sampled by next-event estimation (shadow rays to infinity) and
+    weighted by MIS where BSDF-sampled rays reach it."""
     sc = scenes.cornell(width=48, height=40, spp=2, max_depth=5, seed=11)
     sc["camera"] = dict(sc["camera"], position=(0.5, 0.6, -2.2), fov=1.2)  # box plus surrounding sky
-    sc["sky"] = dict(enabled=True, draw_sun=True, dir=(0.2, 0.3, -1.0), angular_diameter=32.0, intensity=1000.0,
-                     turbidity=4.0)
+    sc["sky"] = dict(enabled=True, draw_sun=True, dir=(0.2, 0.3, -1.0), angular_diameter=32.0 if not sun_nee else 300.0,
+                     intensity=1000.0, turbidity=4.0, sun_nee=sun_nee)
     rend.UpdateScene(sc)
     rend.RenderFrame()
     ref = _oracle(sc).render()
@@ -449,3 +452,34 @@ def test_pick_ray_matches_oracle(rend):
         assert p["primitive"] == h[0, 3:4].view(np.uint32)[0]
         if p["primitive"] != 0xFFFFFFFF:
             assert np.float32(p["t"]) == h[0, 0]
+
+
+def test_accumulator_checkpoint_resumes_the_render(rend):
+    """A progressive render read back (trt_read_accum), restored into a fresh context
+    (trt_write_accum) and continued equals the uninterrupted render bit for bit: the
+    resumed frames trace the following samples."""
+    from trayracer_amd import Renderer
+    sc = scenes.cornell(width=40, height=32, spp=2, max_depth=4, seed=17)
+    rend.UpdateScene(sc)
+    for _ in range(2):
+        rend.RenderFrame()
+    saved = rend.accum().copy()
+    for _ in range(3):
+        rend.RenderFrame()
+    full = rend.accum().copy()
+    other = Renderer(0)
+    try:
+        other.UpdateScene(sc)
+        other.write_accum(saved)
+        assert other.SampleCount() == 4
+        np.testing.assert_array_equal(other.color()[..., :3], saved[..., :3] / np.float32(4))
+        for _ in range(3):
+            other.RenderFrame()
+        assert np.array_equal(other.accum().view(np.uint32), full.view(np.uint32))
+        bad = saved.copy()
+        bad[0, 0, 3] = 3.0
+        from trayracer_amd._lib import TrtError
+        with pytest.raises(TrtError, match="same sample count"):
+            other.write_accum(bad)
+    finally:
+        other.close()

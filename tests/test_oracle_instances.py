@@ -195,9 +195,10 @@ def _dome_scene(seed=9):
 def test_two_level_estimators_agree():
     """Light sampling alone, BSDF sampling alone and both under MIS are each unbiased:
     per pixel their means agree within the standard errors, on an instanced scene lit by
-    an instanced emissive dome (one bounce of indirect light, no Russian roulette)."""
+    an instanced emissive dome (long paths: depth 12 with Russian roulette from depth 3,
+    since BSDF sampling alone cannot reach light through the last vertex)."""
     sc = _dome_scene()
-    sc["render"] = dict(sc["render"], width=24, height=16, spp=512, max_depth=3, rr_depth=9)
+    sc["render"] = dict(sc["render"], width=24, height=16, spp=512, max_depth=12, rr_depth=3)
     orc = Oracle(sc)
     assert orc.lights()[0].shape[0] == 320
     r = sc["render"]
@@ -206,7 +207,7 @@ def test_two_level_estimators_agree():
         L = orc.render(render=dict(strategy=strategy))["path_rgb"].astype(np.float64).sum(axis=1)
         L = L.reshape(r["spp"], -1)
         stats.append((L.mean(axis=0), L.std(axis=0) / math.sqrt(r["spp"]), L.mean()))
-    print("image means (MIS, light, BSDF):", [s[2] for s in stats])
+    print("image means (MIS, light, BSDF):", [s_[2] for s_ in stats])
     m0, s0, t0 = stats[0]
     lit = s0 > 0
     assert lit.mean() > 0.5

@@ -142,3 +142,21 @@ def test_config0_reference_fixture_is_the_oracle_image():
         out = orc.render(render=dict(spp=int(d["spp"])), crop=(x0, y0, 3, 3))
         want = d["mean"][y0:y0 + 3, x0:x0 + 3]
         assert np.array_equal(out["image_f64"].astype(np.float32), want), (x0, y0)
+
+
+def test_last_vertex_light_sample_has_full_weight():
+    """With the balance heuristic, a light sample's weight pl / (pl + pb) leaves the rest
+    to the BSDF sample that the extension ray traces; the last vertex traces none, so its
+    light sample must carry full weight: then MIS equals light sampling alone in
+    expectation at any max depth (checked at depth 2 on the Cornell box, where the last
+    bounce's light is a large part of the image)."""
+    from oracle import Oracle
+    from trayracer_amd import scenes
+    sc = scenes.cornell(width=8, height=8, spp=4096, max_depth=2, rr_depth=9, seed=2)
+    orc = Oracle(sc)
+    res = []
+    for strategy in (0, 1):
+        L = orc.render(render=dict(strategy=strategy))["path_rgb"].astype(np.float64).sum(axis=1)
+        res.append((L.mean(), L.std() / math.sqrt(L.size)))
+    (m0, s0), (m1, s1) = res
+    assert abs(m0 - m1) < 4 * math.hypot(s0, s1), res

@@ -135,3 +135,58 @@ def test_integrator_adds_sky_on_escape():
     assert (w > 0.5).all() and (w <= 1.0 + 1e-6).all()
     sc["sky"] = dict(sky, enabled=False)
     assert not Oracle(sc).render()["path_rgb"].any()
+
+
+def _sunlit_cornell(sun_nee, diameter=600.0, spp=256):
+    """The Cornell box (open at z = 0) with the sun shining in through its open side, a
+    large sun disc (10 degrees) so that BSDF sampling alone also finds it. Paths are long
+    (depth 12, Russian roulette from depth 3): the estimators agree on untruncated
+    light transport (BSDF sampling alone cannot reach light through the last vertex,
+    which traces no extension)."""
+    sc = scenes.cornell(width=20, height=16, spp=spp, max_depth=12, rr_depth=3, seed=3)
+    sc["sky"] = dict(enabled=True, draw_sun=True, dir=(0.15, 0.35, -1.0), angular_diameter=diameter,
+                     intensity=1000.0, turbidity=4.0, sun_nee=sun_nee)
+    return sc
+
+
+def test_sun_nee_estimators_agree():
+    """With the sun disc as a light of next-event estimation (the reference's disabled
+    ENABLE_SKY_NEE branch, Tracer/CUDA/GPU/CudaLights.h:24, 66-88), light sampling alone,
+    BSDF sampling alone (the disc reached by rays that miss) and both under MIS converge
+    to the same image mean (the sun is a rare event for BSDF sampling, so means are
+    compared over all paths of a small, many-sample image); the sun then lights diffuse
+    surfaces, which it does not without the option (the reference's setting: the disc is
+    drawn for camera rays only)."""
+    sc = _sunlit_cornell(True, spp=8192)
+    sc["render"] = dict(sc["render"], width=8, height=6)
+    orc = Oracle(sc)
+    res = []
+    for strategy in (0, 1, 2):
+        L = orc.render(render=dict(strategy=strategy))["path_rgb"].astype(np.float64).sum(axis=1)
+        res.append((L.mean(), L.std() / math.sqrt(L.size)))
+    print("means (MIS, light, BSDF):", res)
+    m0, s0 = res[0]
+    for m, se in res[1:]:
+        assert abs(m - m0) < 4.0 * math.hypot(se, s0), res
+    off = Oracle(dict(sc, sky=dict(sc["sky"], sun_nee=False))).render()["path_rgb"].astype(np.float64)
+    assert m0 > 1.2 * off.sum(axis=1).mean()  # the sun adds light
+
+
+def test_sun_selection_weight():
+    """The sun's selection weight is the disc's extra radiance at its centre x the cone's
+    solid angle 2 pi (1 - cos(diameter)) x the squared half-diagonal of the scene's bounds
+    (numpy), and next-event estimation picks it with probability weight / (weight +
+    triangle power)."""
+    sc = _sunlit_cornell(True, diameter=32.0)
+    orc = Oracle(sc)
+    sun = np.array(sc["sky"]["dir"], np.float64)
+    sun /= np.linalg.norm(sun)
+    c = sun[None].astype(np.float32)
+    extra = (orc.sky_radiance(c, True) - orc.sky_radiance(c, False)).astype(np.float64).sum()
+    omega = 2 * math.pi * (1 - math.cos(math.radians(32.0 / 60.0)))
+    pos = sc["positions"].reshape(-1, 3).astype(np.float64)
+    radius = 0.5 * np.linalg.norm(pos.max(axis=0) - pos.min(axis=0))
+    want = extra * omega * radius ** 2
+    got = orc.sun_power()
+    assert abs(got - want) < 2e-3 * want, (got, want)
+    assert Oracle(_sunlit_cornell(False)).sun_power() == 0.0

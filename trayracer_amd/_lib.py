@@ -64,7 +64,8 @@ class TrtSceneFileInfo(ctypes.Structure):
 
 class TrtSky(ctypes.Structure):
     _fields_ = [("enabled", ctypes.c_uint32), ("draw_sun", ctypes.c_uint32), ("sun_dir", ctypes.c_float * 3),
-                ("angular_diameter", ctypes.c_float), ("intensity", ctypes.c_float), ("turbidity", ctypes.c_float)]
+                ("angular_diameter", ctypes.c_float), ("intensity", ctypes.c_float), ("turbidity", ctypes.c_float),
+                ("sun_nee", ctypes.c_uint32)]
 
 
 class TrtTonemapOpts(ctypes.Structure):
@@ -165,6 +166,7 @@ PROTOTYPES = [
     ("trt_reset", S, [P]),
     ("trt_sample_count", ctypes.c_uint32, [P]),
     ("trt_frame_id", ctypes.c_uint64, [P]),
+    ("trt_write_accum", S, [P, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
     ("trt_read_accum", S, [P, c_float_p]),
     ("trt_read_color", S, [P, c_float_p]),
     ("trt_read_albedo", S, [P, c_float_p]),
@@ -189,6 +191,8 @@ PROTOTYPES = [
     ("trt_denoiser_train", S, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32,
                                c_float_p]),
     ("trt_denoise", S, [P, ctypes.POINTER(TrtDenoiseStats)]),
+    ("trt_denoiser_set_policy", S, [P, ctypes.c_uint32, ctypes.c_uint32]),
+    ("trt_denoise_frame", S, [P, ctypes.POINTER(TrtDenoiseStats), ctypes.POINTER(ctypes.c_uint32)]),
     ("trt_read_denoised", S, [P, c_float_p]),
     ("trt_tonemap", S, [P, ctypes.POINTER(TrtTonemapOpts), c_u8_p]),
     ("trt_write_pfm", S, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),

@@ -1,13 +1,19 @@
 """Time to reach a relative MSE target, plain accumulation vs learned denoising.
 
 relMSE(x, r) = mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2), with r a
-high-spp render of the same view (4096 spp by default). Wall times are measured on
-the GPU with the scene resident: rendering k spp (plus, for the denoised path, one
-denoiser inference). The denoiser is trained first, on pairs (low-spp input, 1024-spp
-target) rendered by the tracer from the 1M-triangle soup, the instanced scene and the
-Cornell box under seeded random cameras; inputs use 1 to 8 spp so the network serves
-every sample count of the progressive render. Training time is reported separately.
-The evaluation view (the Cornell box's own camera) is not among the training views.
+high-spp render of the same view. The reference is rendered as two independent halves
+(different seeds); their mean is r, and a quarter of the halves' relMSE to each other
+estimates r's own error against the converged image (each half has twice r's
+variance), which is reported beside every result and subtracted from the measured
+relMSE for the `*_true` figures. Wall times are measured on the GPU with the scene
+resident: rendering k spp (plus, for the denoised path, one denoiser inference). The
+denoiser is trained first, on pairs (low-spp input, 1024-spp target) rendered by the
+tracer from the benchmark scenes under seeded random cameras; inputs use 1 to 8 spp so
+the network serves every sample count of the progressive render. Training time is
+reported separately. The evaluation views are not among the training views.
+
+The 1M-triangle soup is opaque and its emitters lie inside it, so seen from outside it
+renders black; its views are lit by the daylight sky (the reference's default sky is on).
 """
 import math
 import time
@@ -63,14 +69,28 @@ def _render_spp(rend, spp, seed, frame_spp):
     return time.perf_counter() - t0
 
 
-def build_training_set(rend, width=1920, height=1080, views=(("cornell", 6), ("inst", 4), ("soup", 2)),
+SKY = dict(enabled=True, draw_sun=True, dir=(0.3, 0.8, -0.5), angular_diameter=32.0, intensity=1000.0,
+           turbidity=4.0)
+SOUP_CAMERA = dict(position=(0.5, 0.5, -1.2), target=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov=math.radians(50.0))
+
+
+def make_scene(kind, width=1920, height=1080):
+    """The evaluation scenes: "soup" (the 1M-triangle soup under the sky), "inst" (the
+    1000-instance scene) and "cornell"."""
+    if kind == "soup":
+        return dict(scenes.soup(width=width, height=height), sky=SKY)
+    if kind == "inst":
+        return scenes.instanced(width=width, height=height)
+    return scenes.cornell(width=width, height=height)
+
+
+def build_training_set(rend, width=1920, height=1080, views=(("soup", 4), ("inst", 4)),
                        input_spp=(1, 2, 4, 8), target_spp=1024, seed=0, frame_spp=32, log=print):
-    """One slot per (view, input spp): soup views use 1 spp only (nearly black images)."""
+    """One slot per (view, input spp)."""
     rng = np.random.default_rng(seed)
     plan = []
     for kind, n in views:
-        levels = (1,) if kind == "soup" else tuple(input_spp)
-        plan += [(kind, levels)] * n
+        plan += [(kind, tuple(input_spp))] * n
     total = sum(len(levels) for _, levels in plan)
     rend.denoiser_dataset(total, width, height)
     slot, view = 0, 0
@@ -78,9 +98,7 @@ def build_training_set(rend, width=1920, height=1080, views=(("cornell", 6), ("i
     current = None
     for kind, levels in plan:
         if kind != current:
-            sc = {"soup": scenes.soup, "inst": scenes.instanced, "cornell": scenes.cornell}[kind](
-                width=width, height=height)
-            rend.UpdateScene(sc)
+            rend.UpdateScene(make_scene(kind, width, height))
             current = kind
         rend.SetCamera(random_camera(rng, kind))
         _render_spp(rend, target_spp, seed=5000 + view, frame_spp=frame_spp)
@@ -102,22 +120,27 @@ def train(rend, steps=600, batch=8, crop=128, lr=1e-3, seed=0):
     return time.perf_counter() - t0, losses
 
 
-def time_to_relmse(rend, scene, camera, width=1920, height=1080, target=0.01, ref_spp=4096, max_plain_spp=1024,
+def time_to_relmse(rend, scene, camera, width=1920, height=1080, target=0.01, ref_spp=8192, max_plain_spp=1024,
                    max_denoise_spp=64, frame_spp=32, log=print):
-    """Returns a dict with the plain and denoised times to `target` (seconds)."""
+    """Returns a dict with the plain and denoised times to `target` (seconds) and the
+    reference's estimated own error."""
     rend.UpdateScene(dict(scene, render=dict(scene["render"], width=width, height=height)))
     rend.SetCamera(camera)
-    t_ref = _render_spp(rend, ref_spp, seed=999, frame_spp=frame_spp)
-    ref = rend.color()[..., :3].astype(np.float64)
+    t_a = _render_spp(rend, ref_spp // 2, seed=999, frame_spp=frame_spp)
+    ref_a = rend.color()[..., :3].astype(np.float64)
+    t_b = _render_spp(rend, ref_spp // 2, seed=998, frame_spp=frame_spp)
+    ref_b = rend.color()[..., :3].astype(np.float64)
+    ref = 0.5 * (ref_a + ref_b)
+    ref_err = relmse(ref_a, ref_b) / 4.0
     rend.denoise()  # untimed warm-up: first launches load the kernels
-    log(f"reference: {ref_spp} spp in {t_ref:.1f} s")
+    log(f"reference: {ref_spp} spp in {t_a + t_b:.1f} s, estimated own relMSE {ref_err:.5f}")
     plain = []
     k = 1
     while k <= max_plain_spp:
         t = _render_spp(rend, k, seed=17, frame_spp=frame_spp)
         e = relmse(rend.color(), ref)
         plain.append((k, t, e))
-        if e <= target:
+        if e - ref_err <= target:
             break
         k *= 2
     den = []
@@ -129,19 +152,29 @@ def time_to_relmse(rend, scene, camera, width=1920, height=1080, target=0.01, re
         td = time.perf_counter() - t0
         e = relmse(rend.denoised(), ref)
         den.append((k, t + td, e, st["total_ms"]))
-        if e <= target:
+        if e - ref_err <= target:
             break
         k *= 2
-    # plain: the sample count where relMSE crosses the target under the 1/spp law of
-    # Monte Carlo error from the last measured count (between the last two doublings
-    # when it crossed, beyond the budget otherwise), times the measured seconds per
-    # sample at that count. This is never more than the doubling search measured.
+    # plain: the sample count where the reference-corrected relMSE crosses the target
+    # under the 1/spp law of Monte Carlo error from the last measured count (between the
+    # last two doublings when it crossed, beyond the budget otherwise), times the
+    # measured seconds per sample at that count.
     kp, tp, ep = plain[-1]
-    n_star = kp * ep / target
+    ep_true = max(ep - ref_err, 1e-12)
+    n_star = kp * ep_true / target
     t_plain = tp / kp * n_star
-    plain_how = "interpolated (relMSE ~ 1/spp)" if ep <= target else "extrapolated (relMSE ~ 1/spp)"
+    plain_how = "interpolated (relMSE ~ 1/spp)" if ep_true <= target else "extrapolated (relMSE ~ 1/spp)"
     kd, td_, ed, _ = den[-1]
-    t_den = td_ if ed <= target else None
+    reached = ed - ref_err <= target
+    t_den = td_ if reached else None
+    why = None
+    if not reached:
+        why = (f"the denoised image's relMSE at {kd} spp is {ed:.4f} ({ed - ref_err:.4f} after the reference's own "
+               f"{ref_err:.4f}), above the target")
+    if ref_err > 0.5 * target:
+        why = (why + "; " if why else "") + f"the {ref_spp}-spp reference's own error ({ref_err:.4f}) is comparable " \
+            f"to the target, so the figures are uncertain"
     return dict(target=target, plain_time_s=t_plain, plain_spp=n_star, plain_how=plain_how, denoised_time_s=t_den,
                 denoised_spp=kd if t_den is not None else None, speedup=(t_plain / t_den) if t_den else None,
-                plain_curve=plain, denoised_curve=den, reference_spp=ref_spp, reference_s=t_ref)
+                plain_curve=plain, denoised_curve=den, reference_spp=ref_spp, reference_s=t_a + t_b,
+                reference_relmse=ref_err, failure=why)

@@ -93,6 +93,7 @@ struct trt_context {
   uint32_t touch_node_words = 0, touch_tri_words = 0;
   uint32_t n_lights = 0;
   float total_power = 0.f;
+  float scene_radius = 0.f;  // half-diagonal of the world bounds (the sun's selection weight)
   bool use_normals = false;
   bool two_level = false;
   int stack_cap = 1;
@@ -143,6 +144,8 @@ struct trt_context {
   DevBuf dn_feat, dn_denoised, dn_ds_feat, dn_ds_tgt, dn_batch_feat, dn_batch_tgt, dn_crops, dn_io;
   uint32_t dn_ds_n = 0, dn_ds_w = 0, dn_ds_h = 0;
   std::vector<uint8_t> dn_ds_have;  // per slot: bit0 input, bit1 target
+  // denoising policy of the reference's Renderer::DenoiseFrame (Renderer.cpp:546-569)
+  uint32_t dn_enabled = 0, dn_threshold = 0, dn_last_spp = 0;
 };
 
 namespace {
@@ -150,6 +153,13 @@ namespace {
 // Scene, camera or material changes make the last frame's per-path radiance stale for
 // the adjoint, which replays that frame against the current state.
 void invalidate_frame(trt_context* c) { c->has_frame = false; }
+
+// Restarts progressive accumulation (Renderer::Reset, Tracer/Renderer/Renderer.cpp:186-190),
+// and with it the denoiser's record of the sample count it last ran at (Denoiser::Reset).
+void reset_accumulation(trt_context* c) {
+  c->sample_count = 0;
+  c->dn_last_spp = 0;
+}
 
 trt_status fail(trt_context* c, trt_status s, const std::string& msg) {
   if (c) c->err = msg;
@@ -269,7 +279,7 @@ trt_status ensure_frame(trt_context* ctx, uint32_t W, uint32_t H, uint32_t spp) 
     HIPC(hipMemsetAsync(ctx->normal.p, 0, npix * 16, ctx->stream), "clear normal");
     ctx->width = W;
     ctx->height = H;
-    ctx->sample_count = 0;
+    reset_accumulation(ctx);
   }
   // queues: kSegs segments of seg_cap(n) slots (slack for uneven segment fill)
   ctx->seg_cap = seg_cap(n);
@@ -566,8 +576,9 @@ trt_status trt_set_sky(trt_context* ctx, const trt_sky* sky) {
       !(sky->intensity >= 0.f) || !(sky->intensity < 1e30f) || !(sky->turbidity >= 0.f) || !(sky->turbidity < 1e6f))
     return fail(ctx, TRT_INVALID_ARG, "invalid sky parameters");
   ctx->sky = make_sky(sky->enabled != 0, sky->draw_sun != 0, sky->sun_dir, sky->angular_diameter, sky->intensity,
-                      sky->turbidity);
-  ctx->sample_count = 0;
+                      sky->turbidity, sky->sun_nee != 0);
+  ctx->sky.sun_power = sun_selection_power(ctx->sky, ctx->scene_radius);
+  reset_accumulation(ctx);
   invalidate_frame(ctx);
   return TRT_OK;
 }
@@ -774,6 +785,14 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
                                 mesh_cnt.data());
   ctx->n_lights = uint32_t(lt.lights.size());
   ctx->total_power = lt.total_power;
+  {
+    std::vector<uint32_t> mesh_off(ctx->meshes.size());
+    for (size_t i = 0; i < ctx->meshes.size(); ++i) mesh_off[i] = ctx->meshes[i].tri_offset;
+    ctx->scene_radius = scene_radius(ctx->pos.data(), n_tris, ctx->two_level ? inst_orig.data() : nullptr,
+                                     uint32_t(inst_orig.size()), mesh_off.data(), mesh_cnt.data(),
+                                     uint32_t(ctx->meshes.size()));
+    ctx->sky.sun_power = sun_selection_power(ctx->sky, ctx->scene_radius);
+  }
 
   std::vector<DevNode48> nodes48, tlas48;
   if (!pack_nodes(nodes, nodes48) || !pack_nodes(tlas_nodes, tlas48))
@@ -819,7 +838,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   ctx->info.tlas_depth = uint32_t(tlas_depth);
   ctx->info.references = n_refs;
   ctx->built = true;
-  ctx->sample_count = 0;
+  reset_accumulation(ctx);
   invalidate_frame(ctx);
   return TRT_OK;
 }
@@ -843,7 +862,7 @@ trt_status trt_set_camera(trt_context* ctx, const trt_camera* c) {
     return fail(ctx, TRT_INVALID_ARG, "degenerate camera basis");
   ctx->cam = make_camera(p, t, u, c->fov, c->aperture, c->focal_dist, c->distortion, c->bokeh_sides, c->bokeh_rotation);
   ctx->camera_set = true;
-  ctx->sample_count = 0;  // the reference resets accumulation on camera change
+  reset_accumulation(ctx);  // the reference resets accumulation on camera change
   invalidate_frame(ctx);
   return TRT_OK;
 }
@@ -906,7 +925,7 @@ trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* o, trt
 
 trt_status trt_reset(trt_context* ctx) {
   if (!ctx) return TRT_INVALID_ARG;
-  ctx->sample_count = 0;
+  reset_accumulation(ctx);
   return TRT_OK;
 }
 
@@ -925,6 +944,47 @@ trt_status trt_read_accum(trt_context* ctx, float* rgba) { return ctx ? read_pix
 trt_status trt_read_color(trt_context* ctx, float* rgba) { return ctx ? read_pixels(ctx, ctx->color, rgba) : TRT_INVALID_ARG; }
 trt_status trt_read_albedo(trt_context* ctx, float* rgba) { return ctx ? read_pixels(ctx, ctx->albedo, rgba) : TRT_INVALID_ARG; }
 trt_status trt_read_normal(trt_context* ctx, float* rgba) { return ctx ? read_pixels(ctx, ctx->normal, rgba) : TRT_INVALID_ARG; }
+
+// Restores a progressive render from an accumulator read by trt_read_accum: rgb sums and
+// a sample count (alpha, the same whole number in every pixel). The next trt_render of
+// the same film size continues it with the following samples (their keys start at the
+// restored count), so a restored render adds up to the uninterrupted one.
+trt_status trt_write_accum(trt_context* ctx, uint32_t width, uint32_t height, const float* rgba) {
+  if (!ctx || !rgba || width == 0 || height == 0) return TRT_INVALID_ARG;
+  const uint64_t npix = uint64_t(width) * height;
+  if (npix >= (1ull << 30)) return fail(ctx, TRT_INVALID_ARG, "film too large");
+  const float n = rgba[3];
+  if (!(n >= 1.f && n < 4294967296.f) || n != float(uint32_t(n)))
+    return fail(ctx, TRT_INVALID_ARG, "the accumulator's sample count (alpha) must be a positive whole number");
+  for (uint64_t i = 0; i < npix; ++i) {
+    if (rgba[i * 4 + 3] != n) return fail(ctx, TRT_INVALID_ARG, "every pixel must hold the same sample count");
+    if (!finite_all(rgba + i * 4, 3)) return fail(ctx, TRT_INVALID_ARG, "non-finite accumulator value");
+  }
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  if (width != ctx->width || height != ctx->height) {
+    HIPC(ctx->accum.ensure(npix * 16), "alloc accum");
+    HIPC(ctx->color.ensure(npix * 16), "alloc color");
+    HIPC(ctx->albedo.ensure(npix * 16), "alloc albedo");
+    HIPC(ctx->normal.ensure(npix * 16), "alloc normal");
+    HIPC(hipMemsetAsync(ctx->albedo.p, 0, npix * 16, ctx->stream), "clear albedo");
+    HIPC(hipMemsetAsync(ctx->normal.p, 0, npix * 16, ctx->stream), "clear normal");
+    ctx->width = width;
+    ctx->height = height;
+  }
+  std::vector<float> col(npix * 4);
+  const float inv = 1.f / n;
+  for (uint64_t i = 0; i < npix; ++i) {
+    for (int k = 0; k < 3; ++k) col[i * 4 + k] = rgba[i * 4 + k] * inv;
+    col[i * 4 + 3] = 1.f;
+  }
+  HIPC(hipMemcpyAsync(ctx->accum.p, rgba, npix * 16, hipMemcpyHostToDevice, ctx->stream), "upload accum");
+  HIPC(hipMemcpyAsync(ctx->color.p, col.data(), npix * 16, hipMemcpyHostToDevice, ctx->stream), "upload color");
+  HIPC(hipStreamSynchronize(ctx->stream), "upload accum");
+  ctx->sample_count = uint32_t(n);
+  ctx->dn_last_spp = 0;
+  invalidate_frame(ctx);
+  return TRT_OK;
+}
 
 trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths) {
   if (!ctx || !rgb) return TRT_INVALID_ARG;
@@ -1074,7 +1134,7 @@ trt_status trt_set_material_params(trt_context* ctx, const float* params, uint32
               ctx->stream),
        "upload sampling materials");
   HIPC(hipStreamSynchronize(ctx->stream), "upload materials");
-  ctx->sample_count = 0;
+  reset_accumulation(ctx);
   invalidate_frame(ctx);
   return TRT_OK;
 }
@@ -1312,7 +1372,37 @@ trt_status trt_denoiser_train(trt_context* ctx, uint32_t steps, uint32_t batch, 
   return TRT_OK;
 }
 
-trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* st) {
+static trt_status denoise_impl(trt_context* ctx, trt_denoise_stats* st, float blend);
+
+trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* st) { return denoise_impl(ctx, st, 0.f); }
+
+trt_status trt_denoiser_set_policy(trt_context* ctx, uint32_t enabled, uint32_t sample_threshold) {
+  if (!ctx) return TRT_INVALID_ARG;
+  ctx->dn_enabled = enabled ? 1u : 0u;
+  ctx->dn_threshold = sample_threshold;
+  return TRT_OK;
+}
+
+trt_status trt_denoise_frame(trt_context* ctx, trt_denoise_stats* st, uint32_t* ran) {
+  if (!ctx) return TRT_INVALID_ARG;
+  if (ran) *ran = 0;
+  if (st) *st = trt_denoise_stats{};
+  // Renderer::ShouldDenoise: enabled, path tracing or ambient occlusion, enough samples
+  const uint32_t mode = ctx->last_opts.mode;
+  const bool mode_ok = mode == kModePathTracing || mode == kModeAmbientOcclusion || mode == kModeAmbientOcclusionShading;
+  if (!ctx->dn_enabled || !mode_ok || ctx->sample_count == 0 || ctx->sample_count < ctx->dn_threshold) return TRT_OK;
+  // Renderer::DenoiseFrame: again once the sample count reaches 1.5 x the last denoised one
+  const uint32_t again = ctx->dn_last_spp + (ctx->dn_last_spp >> 1);
+  if (ctx->dn_last_spp != 0 && ctx->sample_count < again) return TRT_OK;
+  // Denoiser::DenoiseFrame: blend factor 1 / (spp + 1) towards the noisy input
+  const trt_status s = denoise_impl(ctx, st, 1.f / float(ctx->sample_count + 1));
+  if (s != TRT_OK) return s;
+  ctx->dn_last_spp = ctx->sample_count;
+  if (ran) *ran = 1;
+  return TRT_OK;
+}
+
+static trt_status denoise_impl(trt_context* ctx, trt_denoise_stats* st, float blend) {
   trt_status s = need_unet(ctx);
   if (s != TRT_OK) return s;
   if (ctx->width == 0 || ctx->sample_count == 0) return fail(ctx, TRT_INVALID_ARG, "nothing rendered yet");
@@ -1330,6 +1420,7 @@ trt_status trt_denoise(trt_context* ctx, trt_denoise_stats* st) {
   HIPC(ctx->unet->forward(ctx->dn_feat.as<uint16_t>(), 1, int(ctx->height), int(ctx->width), ctx->stream,
                           ctx->dn_denoised.as<float4>()),
        "forward");
+  if (blend > 0.f) dn_blend(ctx->dn_denoised.as<float4>(), ctx->color.as<float4>(), blend, P, ctx->stream);
   HIPC(hipEventRecord(ev[2], ctx->stream), "event");
   HIPC(hipStreamSynchronize(ctx->stream), "denoise");
   if (st) {

@@ -51,6 +51,8 @@ void dn_upcat_bwd(const float* dout, float* dcoarse, float* dskip, int N, int H,
 void dn_features(const float4* color, const float4* albedo, const float4* normal, uint16_t* out, size_t P,
                  hipStream_t s);
 void dn_target(const float4* color, float* out, size_t P, hipStream_t s);
+// den = (1 - b) den + b color (rgb; alpha kept)
+void dn_blend(float4* den, const float4* color, float b, size_t P, hipStream_t s);
 void dn_gather_crop(const uint16_t* feat, const float* tgt, int Hs, int Ws, const int4* crops, int B, int S,
                     uint16_t* fo, float* to, hipStream_t s);
 int dn_loss(const float* Y, const uint16_t* X, const float* T, size_t P, float* dY, float* partial, hipStream_t s);

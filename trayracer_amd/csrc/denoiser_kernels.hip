@@ -616,6 +616,14 @@ __global__ void k_features(const float4* __restrict__ color, const float4* __res
   for (int k = 0; k < 16; ++k) o[k] = f2bf(f[k]);
 }
 
+// denoised = (1 - b) denoised + b noisy (OptiX's blend factor, Tracer/Optix/Denoiser.cpp:108)
+__global__ void k_blend(float4* __restrict__ den, const float4* __restrict__ color, float b, size_t P) {
+  const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const float4 d = den[p], c = color[p];
+  den[p] = make_float4(fmaf(b, c.x - d.x, d.x), fmaf(b, c.y - d.y, d.y), fmaf(b, c.z - d.z, d.z), d.w);
+}
+
 // training target: log(1+radiance) of the reference image
 __global__ void k_target(const float4* __restrict__ color, float* __restrict__ out, size_t P) {
   const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -864,6 +872,9 @@ void dn_upcat_bwd(const float* dout, float* dcoarse, float* dskip, int N, int H,
 void dn_features(const float4* color, const float4* albedo, const float4* normal, uint16_t* out, size_t P,
                  hipStream_t s) {
   hipLaunchKernelGGL(k_features, dim3(blocks_for(P)), dim3(256), 0, s, color, albedo, normal, out, P);
+}
+void dn_blend(float4* den, const float4* color, float b, size_t P, hipStream_t s) {
+  hipLaunchKernelGGL(k_blend, dim3(blocks_for(P)), dim3(256), 0, s, den, color, b, P);
 }
 void dn_target(const float4* color, float* out, size_t P, hipStream_t s) {
   hipLaunchKernelGGL(k_target, dim3(blocks_for(P)), dim3(256), 0, s, color, out, P);

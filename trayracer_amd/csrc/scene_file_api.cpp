@@ -163,6 +163,7 @@ trt_status trt_load_scene_file(trt_context* ctx, const char* path, const trt_bui
   sky.angular_diameter = local.sky_angular_diameter;
   sky.intensity = local.sky_intensity;
   sky.turbidity = local.sky_turbidity;
+  sky.sun_nee = 0;  // the reference's setting (its scene files carry no such switch)
   if ((s = trt_set_sky(ctx, &sky)) != TRT_OK) return s;
   if (info) *info = local;
   return TRT_OK;

@@ -96,11 +96,15 @@ struct ShadeOut {
   float vdepth;
 };
 
-// Solid-angle pdf of light sampling towards an emitter hit at distance t.
-TRT_FI float light_pdf_hit(const SceneView& s, V3 le, float area, float cos_l, float t) {
-  const float pick = light_power(le, area) / s.total_power;
+// Solid-angle pdf of light sampling towards an emitter hit at distance t (total: the
+// selection weight of all lights, the sun's included).
+TRT_FI float light_pdf_hit(V3 le, float area, float cos_l, float t, float total) {
+  const float pick = light_power(le, area) / total;
   return pick * (t * t) / (area * cos_l);
 }
+
+// Selection weight of the sun disc in next-event estimation (0 when it is not a light).
+TRT_FI float sun_weight(const SceneView& s) { return s.sky.sun_nee ? s.sky.sun_power : 0.f; }
 
 // Adjoint extras of one vertex (path-replay backpropagation, Vicini et al. 2021):
 // derivatives of the vertex's terms with respect to its material's differentiable
@@ -128,7 +132,18 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
   if (h.prim == kNoHit) {  // the ray left the scene: sky radiance (PathTracing.h:40-55)
     if (s.sky.enabled) {
       const bool sun = s.sky.draw_sun != 0u && (depth == 0 || (ps.flags & kFlagSpecular));
-      const V3 sky = sky_radiance(s.sky, ps.d, sun);
+      V3 sky = sky_radiance(s.sky, ps.d, sun);
+      // a BSDF-sampled ray that reaches the sun disc when the disc is also a light:
+      // its radiance with the MIS weight against the sun's next-event estimate
+      if (!sun && s.sky.sun_nee && rp.strategy != 1u) {
+        const V3 ls = sun_radiance(s.sky, ps.d);
+        if (!is_black(ls)) {
+          const float sw = sun_weight(s);
+          const float pl = (sw / (s.total_power + sw)) * s.sky.sun_pdf;
+          const float w = rp.strategy == 2u ? 1.f : ps.pdf / (ps.pdf + pl);
+          sky = sky + ls * w;
+        }
+      }
       out.L = clamp_contrib(ps.T * sky, rp.clamp);
       if (depth == 0) out.albedo = sky;
     }
@@ -143,9 +158,22 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
   Pcg32 rng = rng_for(rp.seed, pixel, sample, depth);
   float u[8];
   for (int i = 0; i < 8; ++i) u[i] = rng_float(rng);
-  const bool nee = s.n_lights > 0 && s.total_power > 0.f;
+  // light choice: the sun disc with probability sun / total when it is a light (u[0]
+  // rescaled for the triangle choice otherwise), else an emissive triangle by power
+  const float sun_w = sun_weight(s);
+  const float total = s.total_power + sun_w;
+  const bool nee = (s.n_lights > 0 && s.total_power > 0.f) || sun_w > 0.f;
+  bool pick_sun = false;
   LightTri L{};
-  if (nee) L = s.lights[alias_sample(s.alias_prob, s.alias_idx, s.n_lights, u[0], u[1])];
+  if (nee) {
+    float u0 = u[0];
+    if (sun_w > 0.f) {
+      const float ps_sun = sun_w / total;
+      pick_sun = u0 < ps_sun;
+      u0 = fmin((u0 - ps_sun) / (1.f - ps_sun), 0x1.fffffep-1f);
+    }
+    if (!pick_sun) L = s.lights[alias_sample(s.alias_prob, s.alias_idx, s.n_lights, u0, u[1])];
+  }
 
   const SurfacePoint sp = surface_at(s, h, ps.o, ps.d);
   const Material& m = s.materials[sp.mat];
@@ -163,7 +191,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
       if (depth == 0 || (ps.flags & kFlagSpecular)) {
         out.L = clamp_contrib(ps.T * le, rp.clamp);
       } else if (rp.strategy != 1u) {
-        const float pl = light_pdf_hit(s, le, sp.area, cos_l, h.t);
+        const float pl = light_pdf_hit(le, sp.area, cos_l, h.t, total);
         const float w = rp.strategy == 2u ? 1.f : ps.pdf / (ps.pdf + pl);
         out.L = clamp_contrib(ps.T * le * w, rp.clamp);
       }
@@ -195,43 +223,64 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
     bd.model = bp.model;
   }
 
-  // next-event estimation
+  // next-event estimation: a point on the chosen triangle, or a direction in the sun's cone
   if (nee && rp.strategy != 2u) {
-    float b1, b2;
-    sample_triangle(u[2], u[3], b1, b2);
-    const V3 p0 = v3(L.p0[0], L.p0[1], L.p0[2]);
-    const V3 e1 = v3(L.e1[0], L.e1[1], L.e1[2]);
-    const V3 e2 = v3(L.e2[0], L.e2[1], L.e2[2]);
-    const V3 q = v3(fma(b2, e2.x, fma(b1, e1.x, p0.x)), fma(b2, e2.y, fma(b1, e1.y, p0.y)),
-                    fma(b2, e2.z, fma(b1, e1.z, p0.z)));
-    const V3 dir = q - sp.p;
-    const float dist2 = dot(dir, dir);
-    const float dist = sqrt(dist2);
-    if (dist > 0.f) {
-      const V3 wl = dir * (1.f / dist);
-      const float cos_l = -dot(wl, v3(L.n[0], L.n[1], L.n[2]));
-      if (cos_l > 0.f && dot(wl, sp.ng) > 0.f) {
-        const V3 wi = to_local(fr, wl);
-        if (wi.z > 0.f) {
-          const T3<float> f = bsdf_f(bp, wo, wi);
-          const float pl = (L.power / s.total_power) * dist2 / (L.area * cos_l);
-          const float pb = bsdf_pdf(bs, wo, wi);
-          const float w = rp.strategy == 1u ? 1.f : pl / (pl + pb);
-          const float k = wi.z * (w / pl);
-          const V3 c = v3(ps.T.x * f.x * L.le[0] * k, ps.T.y * f.y * L.le[1] * k, ps.T.z * f.z * L.le[2] * k);
-          if (!is_black(c)) {
-            out.has_shadow = 1;
-            out.s_o = sp.p;
-            out.s_d = wl;
-            out.s_tmax = dist - 2.f * rp.eps;
-            out.s_contrib = clamp_contrib(c, rp.clamp);
-            if (ADJ) {
-              const T3<Dual<kNumParams>> fd = bsdf_f(bd, wo, wi);
-              const float kx = delta.x * ps.T.x * L.le[0] * k, ky = delta.y * ps.T.y * L.le[1] * k,
-                          kz = delta.z * ps.T.z * L.le[2] * k;
-              for (int j = 0; j < kNumParams; ++j)
-                adj->nee_g[j] = (kx * fd.x.d[j] + ky * fd.y.d[j]) + kz * fd.z.d[j];
-            }
+    V3 wl = v3(0.f), lrad = v3(0.f);
+    float pl = 0.f, s_tmax = 0.f;
+    bool ok = false;
+    if (pick_sun) {
+      wl = sample_sun(s.sky, u[2], u[3]);
+      if (dot(wl, sp.ng) > 0.f) {
+        lrad = sun_radiance(s.sky, wl);
+        pl = (sun_w / total) * s.sky.sun_pdf;
+        s_tmax = 1e30f;
+        ok = !is_black(lrad);
+      }
+    } else {
+      float b1, b2;
+      sample_triangle(u[2], u[3], b1, b2);
+      const V3 p0 = v3(L.p0[0], L.p0[1], L.p0[2]);
+      const V3 e1 = v3(L.e1[0], L.e1[1], L.e1[2]);
+      const V3 e2 = v3(L.e2[0], L.e2[1], L.e2[2]);
+      const V3 q = v3(fma(b2, e2.x, fma(b1, e1.x, p0.x)), fma(b2, e2.y, fma(b1, e1.y, p0.y)),
+                      fma(b2, e2.z, fma(b1, e1.z, p0.z)));
+      const V3 dir = q - sp.p;
+      const float dist2 = dot(dir, dir);
+      const float dist = sqrt(dist2);
+      if (dist > 0.f) {
+        wl = dir * (1.f / dist);
+        const float cos_l = -dot(wl, v3(L.n[0], L.n[1], L.n[2]));
+        if (cos_l > 0.f && dot(wl, sp.ng) > 0.f) {
+          lrad = v3(L.le[0], L.le[1], L.le[2]);
+          pl = (L.power / total) * dist2 / (L.area * cos_l);
+          s_tmax = dist - 2.f * rp.eps;
+          ok = true;
+        }
+      }
+    }
+    if (ok) {
+      const V3 wi = to_local(fr, wl);
+      if (wi.z > 0.f) {
+        const T3<float> f = bsdf_f(bp, wo, wi);
+        const float pb = bsdf_pdf(bs, wo, wi);
+        // balance heuristic against BSDF sampling, whose sample would be traced by the
+        // extension ray; the last vertex traces none, so there the light sample stands
+        // alone (a partial weight would drop the BSDF share of the last bounce's light)
+        const float w = (rp.strategy == 1u || depth + 1 >= rp.max_depth) ? 1.f : pl / (pl + pb);
+        const float k = wi.z * (w / pl);
+        const V3 c = v3(ps.T.x * f.x * lrad.x * k, ps.T.y * f.y * lrad.y * k, ps.T.z * f.z * lrad.z * k);
+        if (!is_black(c)) {
+          out.has_shadow = 1;
+          out.s_o = sp.p;
+          out.s_d = wl;
+          out.s_tmax = s_tmax;
+          out.s_contrib = clamp_contrib(c, rp.clamp);
+          if (ADJ) {
+            const T3<Dual<kNumParams>> fd = bsdf_f(bd, wo, wi);
+            const float kx = delta.x * ps.T.x * lrad.x * k, ky = delta.y * ps.T.y * lrad.y * k,
+                        kz = delta.z * ps.T.z * lrad.z * k;
+            for (int j = 0; j < kNumParams; ++j)
+              adj->nee_g[j] = (kx * fd.x.d[j] + ky * fd.y.d[j]) + kz * fd.z.d[j];
           }
         }
       }

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "scene.h"
+#include "sky.h"
 
 namespace trt {
 
@@ -110,6 +111,47 @@ inline LightTable gather_lights(const float* tri_pos, const uint32_t* tri_mat, u
   t.total_power = float(tot);
   build_alias_table(w, t.prob, t.alias);
   return t;
+}
+
+// Half-diagonal of the world-space bounds of the scene: all vertices of a single-level
+// scene; for instanced scenes the transformed corners of each instance's mesh bounds.
+// Sets the sun's selection weight (sky.h), so the GPU context and the oracle compute
+// it with this same function.
+inline float scene_radius(const float* tri_pos, uint32_t n_tris, const Instance* instances, uint32_t n_inst,
+                          const uint32_t* mesh_tri_offset, const uint32_t* mesh_tri_count, uint32_t n_meshes) {
+  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+  auto grow = [&](V3 p) {
+    const float c[3] = {p.x, p.y, p.z};
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fmin(lo[k], c[k]);
+      hi[k] = fmax(hi[k], c[k]);
+    }
+  };
+  if (!instances) {
+    for (size_t i = 0; i < size_t(n_tris) * 3; ++i) grow(v3(tri_pos[i * 3], tri_pos[i * 3 + 1], tri_pos[i * 3 + 2]));
+  } else {
+    std::vector<float> mb(size_t(n_meshes) * 6);
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+      float ml[3] = {1e30f, 1e30f, 1e30f}, mh[3] = {-1e30f, -1e30f, -1e30f};
+      for (size_t i = size_t(mesh_tri_offset[m]) * 3; i < size_t(mesh_tri_offset[m] + mesh_tri_count[m]) * 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+          ml[k] = fmin(ml[k], tri_pos[i * 3 + k]);
+          mh[k] = fmax(mh[k], tri_pos[i * 3 + k]);
+        }
+      for (int k = 0; k < 3; ++k) {
+        mb[m * 6 + k] = ml[k];
+        mb[m * 6 + 3 + k] = mh[k];
+      }
+    }
+    for (uint32_t i = 0; i < n_inst; ++i) {
+      const float* b = &mb[size_t(instances[i].mesh) * 6];
+      if (b[0] > b[3]) continue;  // empty mesh
+      for (int c = 0; c < 8; ++c)
+        grow(xform_point(instances[i].o2w, v3(b[(c & 1) ? 3 : 0], b[(c & 2) ? 4 : 1], b[(c & 4) ? 5 : 2])));
+    }
+  }
+  if (lo[0] > hi[0]) return 0.f;
+  return 0.5f * length(v3(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]));
 }
 
 }  // namespace trt

@@ -10,11 +10,19 @@
 // radiance. Everything is evaluated with the project's own exp/log/atan (common.h) on
 // both host and device, so the oracle and the GPU agree bit for bit.
 //
-// As in the reference (CudaLights.h:11, ENABLE_SKY_NEE false), the sky is not sampled
-// by next-event estimation: it is reached by BSDF-sampled rays that miss, and the sun
-// disc is drawn only for camera rays and specular bounces (PathTracing.h:40-55).
+// By default, as in the reference (CudaLights.h:11, ENABLE_SKY_NEE false), the sky is
+// not sampled by next-event estimation: it is reached by BSDF-sampled rays that miss,
+// and the sun disc is drawn only for camera rays and specular bounces
+// (PathTracing.h:40-55). With `sun_nee` the sun disc becomes a light of next-event
+// estimation, chosen against the emissive triangles by energy as the reference's
+// disabled branch does (CudaLights.h:24, 66-88): its direction is sampled uniformly in
+// the disc's cone, and BSDF-sampled rays that reach the disc add its radiance with the
+// balance-heuristic MIS weight. Its selection weight is the disc's radiance x solid
+// angle x the squared radius of the scene's bounds (the power a sphere around the scene
+// intercepts, in the triangles' radiance x area units).
 #pragma once
 #include "common.h"
+#include "sampling.h"
 
 namespace trt {
 
@@ -28,6 +36,9 @@ struct SkyParams {
   float beta_m[3];     // total Mie scattering x Mie coefficient, per primary
   float mix_w;         // saturate((1 - cos(sun, up))^5): weight of the in-scattering correction
   float gamma;         // final exponent 1 / (1 + sunfade)
+  uint32_t sun_nee;    // 1: the sun disc is a light of next-event estimation
+  float sun_pdf;       // 1 / solid angle of the disc's cone (uniform direction sampling)
+  float sun_power;     // selection weight against the emissive triangles (set per scene)
 };
 
 constexpr float kSkyPi = 3.14159265358979f;
@@ -51,10 +62,11 @@ TRT_FI float smoothstep_sky(float e0, float e1, float x) {
 // angular_diameter in arc minutes (the reference's unit), intensity of the sun,
 // turbidity of the atmosphere.
 TRT_HD inline SkyParams make_sky(bool enabled, bool draw_sun, const float dir[3], float angular_diameter,
-                                 float intensity, float turbidity) {
+                                 float intensity, float turbidity, bool sun_nee = false) {
   SkyParams s{};
   s.enabled = enabled ? 1u : 0u;
   s.draw_sun = draw_sun ? 1u : 0u;
+  s.sun_nee = (enabled && draw_sun && sun_nee) ? 1u : 0u;
   V3 d = v3(dir[0], dir[1], dir[2]);
   const float l = length(d);
   d = l > 0.f ? d * (1.f / l) : v3(0.f, 1.f, 0.f);
@@ -65,6 +77,12 @@ TRT_HD inline SkyParams make_sky(bool enabled, bool draw_sun, const float dir[3]
   float sn, cs;
   sincos_rad(diam, sn, cs);
   s.cos_disc = cs;
+  // the disc's cone: 1 - cos = 2 sin^2(diam / 2), free of cancellation for small discs
+  float sh, ch;
+  sincos_rad(0.5f * diam, sh, ch);
+  const float one_m_cos = 2.f * sh * sh;
+  s.sun_pdf = one_m_cos > 0.f ? 1.f / (2.f * kSkyPi * one_m_cos) : 0.f;
+  if (!(one_m_cos > 0.f)) s.sun_nee = 0u;
   const float cos_up = d.y;
   // sun intensity falling off below the horizon ("earth shadow"): cutoff pi/1.95, steepness 1.5
   s.sun_e = intensity * fmax(0.f, 1.f - exp(-((kSkyPi / 1.95f - acos_sky(cos_up)) / 1.5f)));
@@ -126,6 +144,32 @@ TRT_FI V3 sky_radiance(const SkyParams& s, V3 dir, bool sun) {
     out[i] = pow(tm, s.gamma);
   }
   return v3(out[0], out[1], out[2]);
+}
+
+// Radiance the sun disc adds along `dir` (zero outside the disc).
+TRT_FI V3 sun_radiance(const SkyParams& s, V3 dir) {
+  const V3 sd = v3(s.sun_dir[0], s.sun_dir[1], s.sun_dir[2]);
+  if (!(dot(dir, sd) > s.cos_disc)) return v3(0.f);
+  const V3 a = sky_radiance(s, dir, true), b = sky_radiance(s, dir, false);
+  return v3(fmax(a.x - b.x, 0.f), fmax(a.y - b.y, 0.f), fmax(a.z - b.z, 0.f));
+}
+
+// Direction uniformly distributed in the disc's cone (pdf = sun_pdf).
+TRT_FI V3 sample_sun(const SkyParams& s, float u0, float u1) {
+  const float omc = 1.f / (2.f * kSkyPi * s.sun_pdf);  // 1 - cos of the cone
+  const float z = 1.f - u0 * omc;
+  const float r = sqrt(fmax(u0 * omc * (2.f - u0 * omc), 0.f));  // sin = sqrt(1 - z^2)
+  float sn, cs;
+  sincos_turns(u1, sn, cs);
+  const Frame f = make_frame(v3(s.sun_dir[0], s.sun_dir[1], s.sun_dir[2]));
+  return normalize(to_world(f, v3(r * cs, r * sn, z)));
+}
+
+// Host: the sun's selection weight for a scene whose bounds have half-diagonal `radius`.
+inline float sun_selection_power(const SkyParams& s, float radius) {
+  if (!s.sun_nee) return 0.f;
+  const V3 l = sun_radiance(s, v3(s.sun_dir[0], s.sun_dir[1], s.sun_dir[2]));
+  return (l.x + l.y + l.z) / s.sun_pdf * (radius * radius);
 }
 
 }  // namespace trt

@@ -173,10 +173,13 @@ class Renderer:
 
     def SetSky(self, sky):
         """sky: dict(enabled, draw_sun, dir, angular_diameter (arc minutes), intensity,
-        turbidity) as in the reference's scene files (Tracer/Renderer/Sky.h)."""
+        turbidity) as in the reference's scene files (Tracer/Renderer/Sky.h), and sun_nee
+        (the sun disc as a light of next-event estimation; off by default, as in the
+        reference)."""
         c = _lib.TrtSky(int(sky.get("enabled", True)), int(sky.get("draw_sun", True)),
                         (ctypes.c_float * 3)(*sky.get("dir", (0.0, 1.0, 0.0))), float(sky.get("angular_diameter", 32.0)),
-                        float(sky.get("intensity", 1000.0)), float(sky.get("turbidity", 4.0)))
+                        float(sky.get("intensity", 1000.0)), float(sky.get("turbidity", 4.0)),
+                        int(sky.get("sun_nee", False)))
         self._check(self.lib.trt_set_sky(self.ctx, ctypes.byref(c)))
 
     # -- camera / settings -----------------------------------------------------------
@@ -230,6 +233,13 @@ class Renderer:
 
     def accum(self):
         return self._pixels(self.lib.trt_read_accum)
+
+    def write_accum(self, rgba):
+        """Restores a progressive render from an accumulator read by accum() (same film
+        size as the settings); the next RenderFrame continues it."""
+        a = np.ascontiguousarray(rgba, np.float32)
+        H, W = a.shape[0], a.shape[1]
+        self._check(self.lib.trt_write_accum(self.ctx, W, H, _fp(a)))
 
     def albedo(self):
         return self._pixels(self.lib.trt_read_albedo)
@@ -309,6 +319,18 @@ class Renderer:
 
     def denoised(self):
         return self._pixels(self.lib.trt_read_denoised)
+
+    def SetDenoiser(self, enabled=True, sample_threshold=0):
+        """The reference's Denoiser::SetEnabled / SetSampleTreshold."""
+        self._check(self.lib.trt_denoiser_set_policy(self.ctx, int(enabled), int(sample_threshold)))
+
+    def DenoiseFrame(self):
+        """Renderer::DenoiseFrame: denoises (blended by 1 / (spp + 1)) when the policy
+        says so for the current sample count; returns the stats, or None when it did not."""
+        st = _lib.TrtDenoiseStats()
+        ran = ctypes.c_uint32()
+        self._check(self.lib.trt_denoise_frame(self.ctx, ctypes.byref(st), ctypes.byref(ran)))
+        return st.as_dict() if ran.value else None
 
     def denoiser_forward(self, features):
         f = np.ascontiguousarray(features, np.float32)
