@@ -321,6 +321,19 @@ trt_status trt_read_path_radiance(trt_context* ctx, float* rgb, uint64_t n_paths
 trt_status trt_trace_closest(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst);
 trt_status trt_trace_any(trt_context* ctx, const float* rays, uint32_t n, uint8_t* occluded);
 
+/* trt_trace_closest without device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory during the call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it). Same arguments, checks and
+ * results as trt_trace_closest, at most TRT_NOALLOC_MAX_RAYS rays per call
+ * (TRT_INVALID_ARG beyond). Its staging (the ray records, their queue form and the hit
+ * records, 96 bytes per ray) is one fixed device array per GPU, held until the process
+ * ends, instead of context buffers grown on demand; the context's own buffers (queue
+ * counters, traversal stack spill area) come from trt_create and trt_build. Every call
+ * on a GPU shares that GPU's array: calls that overlap on one GPU, from any context or
+ * thread, race and must be serialised by the caller. */
+#define TRT_NOALLOC_MAX_RAYS 65536u
+trt_status trt_trace_closest_noalloc(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst);
+
 /* Ray pick (replaces DeviceRenderer::PickRay, Tracer/Renderer/DeviceRenderer.cpp:71-93, with the
  * RayPick ray-gen mode, Tracer/Optix/GPU/optix.cu:345-377): camera ray through the
  * centre of pixel (x, y) (y = 0 bottom row), no jitter. */

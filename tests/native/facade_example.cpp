@@ -35,6 +35,7 @@ int main(int argc, char** argv) {
     for (size_t i = 0; i < c.size(); i += 4)
       for (int k = 0; k < 3; ++k) sum[k] += c[i + k];
     const trt_pick_result pick = r.PickRay(64, 48);
+    const uint32_t samples = r.SampleCount();
     // an invalid setting is reported as an exception, not an exit
     bool caught = false;
     try {
@@ -46,7 +47,7 @@ int main(int argc, char** argv) {
       caught = e.status() == TRT_INVALID_ARG;
     }
     std::printf("ok models=%u instances=%u samples=%u rays=%llu mean=%.9g,%.9g,%.9g pick_prim=%u pick_t=%.9g error_caught=%d\n",
-                info.n_models, info.n_instances, r.SampleCount(), static_cast<unsigned long long>(rays),
+                info.n_models, info.n_instances, samples, static_cast<unsigned long long>(rays),
                 sum[0] / (c.size() / 4), sum[1] / (c.size() / 4), sum[2] / (c.size() / 4), pick.primitive, pick.t,
                 caught ? 1 : 0);
     return 0;

@@ -177,12 +177,12 @@ def test_denoising_policy_and_blend(rend):
     assert rend.DenoiseFrame() is None  # disabled
     rend.SetDenoiser(True, 4)
     ran = []
-    for _ in range(14):
+    for _ in range(13):
         rend.RenderFrame()
         if rend.DenoiseFrame() is not None:
             ran.append(rend.SampleCount())
     assert ran == [4, 6, 9, 13], ran
-    # blend: against the unblended network output of the same frame
+    # blend: against the unblended network output of the same frame (13 samples)
     blended = rend.denoised()[..., :3].astype(np.float64)
     spp = rend.SampleCount()
     rend.denoise()  # plain trt_denoise: no blend
@@ -190,6 +190,8 @@ def test_denoising_policy_and_blend(rend):
     col = rend.color()[..., :3].astype(np.float64)
     b = 1.0 / (spp + 1)
     np.testing.assert_allclose(blended, (1 - b) * raw + b * col, rtol=1e-5, atol=1e-6)
+    rend.RenderFrame()
+    assert rend.DenoiseFrame() is None  # 14 < 1.5 x 13
     rend.Reset()
     rend.RenderFrame()
     assert rend.DenoiseFrame() is None  # 1 sample after the restart

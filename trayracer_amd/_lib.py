@@ -166,6 +166,7 @@ PROTOTYPES = [
     ("trt_reset", S, [P]),
     ("trt_sample_count", ctypes.c_uint32, [P]),
     ("trt_frame_id", ctypes.c_uint64, [P]),
+    ("trt_trace_closest_noalloc", S, [P, c_float_p, ctypes.c_uint32, c_float_p, c_u32_p]),
     ("trt_write_accum", S, [P, ctypes.c_uint32, ctypes.c_uint32, c_float_p]),
     ("trt_read_accum", S, [P, c_float_p]),
     ("trt_read_color", S, [P, c_float_p]),
@@ -214,6 +215,9 @@ STATUS = {0: "TRT_OK", 1: "TRT_INVALID_ARG", 2: "TRT_OUT_OF_MEMORY", 3: "TRT_DEV
           5: "TRT_UNSUPPORTED"}
 
 _lib = None
+
+
+TRT_NOALLOC_MAX_RAYS = 65536  # include/trt.h
 
 
 class TrtError(RuntimeError):

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <chrono>
 #include <cstring>
@@ -1084,6 +1085,102 @@ static trt_status trace_common(trt_context* ctx, const float* rays, uint32_t n, 
 trt_status trt_trace_closest(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst) {
   if (!hits) return TRT_INVALID_ARG;
   return trace_common(ctx, rays, n, hits, inst, nullptr, false);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Allocation-free closest-hit queries (trt_trace_closest_noalloc): the per-call staging
+// of trt_trace_closest (ray records, their structure-of-arrays form, hit records) lives
+// in one file-scope device array per GPU instead of context buffers grown on demand.
+// ---------------------------------------------------------------------------------
+namespace {
+
+constexpr uint32_t kNoAllocMaxRays = TRT_NOALLOC_MAX_RAYS;
+// per ray: 32 B caller record, 16 + 16 B origin/direction rows, 16 B hit, 4 B instance
+constexpr size_t kNoAllocBytes = size_t(kNoAllocMaxRays) * 96;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (`device`), looked up once per
+// device (each GPU holds its own copy of the array).
+hipError_t scratch_ptr(int device, uint8_t** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = p;
+  }
+  *out = static_cast<uint8_t*>(g_scratch[device]);
+  return hipSuccess;
+}
+
+// Ray records (8 floats: origin, tmin, direction, tmax) -> the (origin, tmin) and
+// (direction, tmax) rows of a depth-0 queue whose slot i is ray i, and the queue's
+// per-segment counts (kSegs segments of `per` slots).
+__global__ void k_noalloc_prepare(const float4* __restrict__ rec, uint32_t n, uint32_t per, float4* o_tmin,
+                                  float4* d_tmax, uint32_t* seg_cnt) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    o_tmin[i] = rec[2 * size_t(i)];
+    d_tmax[i] = rec[2 * size_t(i) + 1];
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kSegs) {
+    const uint64_t first = uint64_t(threadIdx.x) * per;
+    seg_cnt[threadIdx.x * kLine] = first >= n ? 0u : uint32_t(n - first < per ? n - first : per);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+trt_status trt_trace_closest_noalloc(trt_context* ctx, const float* rays, uint32_t n, float* hits, uint32_t* inst) {
+  if (!ctx || (n && (!rays || !hits))) return TRT_INVALID_ARG;
+  if (!ctx->built) return fail(ctx, TRT_NOT_BUILT, "scene not built");
+  if (n > kNoAllocMaxRays)
+    return fail(ctx, TRT_INVALID_ARG, "trt_trace_closest_noalloc takes at most " + std::to_string(kNoAllocMaxRays) +
+                                          " rays per call");
+  if (n == 0) return TRT_OK;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* r = &rays[size_t(i) * 8];
+    if (!finite_all(r, 3) || !finite_all(r + 4, 3) || r[3] != r[3] || r[7] != r[7])
+      return fail(ctx, TRT_INVALID_ARG, "ray " + std::to_string(i) + " has a non-finite origin or direction or a NaN bound");
+    if (!(r[3] >= 0.f)) return fail(ctx, TRT_INVALID_ARG, "ray " + std::to_string(i) + " has a negative tmin");
+  }
+  HIPC(hipSetDevice(ctx->device), "hipSetDevice");
+  uint8_t* base = nullptr;
+  HIPC(scratch_ptr(ctx->device, &base), "scratch address");
+  float4* rec = reinterpret_cast<float4*>(base);
+  float4* o_tmin = rec + 2 * size_t(kNoAllocMaxRays);
+  float4* d_tmax = o_tmin + kNoAllocMaxRays;
+  float4* thit = d_tmax + kNoAllocMaxRays;
+  uint32_t* tinst = reinterpret_cast<uint32_t*>(thit + kNoAllocMaxRays);
+  hipStream_t stm = ctx->stream;
+  uint32_t* ctr = ctx->counters.as<uint32_t>();
+  HIPC(hipMemcpyAsync(rec, rays, size_t(n) * 32, hipMemcpyHostToDevice, stm), "upload rays");
+  launch_zero_counters(ctr, stm);
+  const uint32_t per = (n + kSegs - 1) / kSegs;
+  hipLaunchKernelGGL(k_noalloc_prepare, dim3((n + 255) / 256), dim3(256), 0, stm, rec, n, per, o_tmin, d_tmax,
+                     ctr + seg_ctr(kSegExt, 0));
+  const SegIn in{ctr + seg_ctr(kSegExt, 0), ctr + seg_ctr(kSegWorkExt, 0), per};
+  launch_extend(accel_of(ctx), RayQueue{o_tmin, d_tmax, nullptr, nullptr}, HitBuf{thit, tinst}, in, ctr,
+                ctx->grid_trace, false, stm);
+  HIPC(hipGetLastError(), "trace launch");
+  uint32_t overflow = 0;
+  HIPC(hipMemcpyAsync(hits, thit, size_t(n) * 16, hipMemcpyDeviceToHost, stm), "read hits");
+  if (inst && ctx->two_level) HIPC(hipMemcpyAsync(inst, tinst, size_t(n) * 4, hipMemcpyDeviceToHost, stm), "read hits");
+  HIPC(hipMemcpyAsync(&overflow, ctr + kCtrOverflow, 4, hipMemcpyDeviceToHost, stm), "read overflow");
+  HIPC(hipStreamSynchronize(stm), "trace");
+  if (overflow) return fail(ctx, TRT_DEVICE, "traversal stack overflow");
+  if (inst && !ctx->two_level)
+    for (uint32_t i = 0; i < n; ++i) inst[i] = kNoInstance;
+  return TRT_OK;
 }
 
 trt_status trt_trace_any(trt_context* ctx, const float* rays, uint32_t n, uint8_t* occluded) {
