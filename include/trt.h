@@ -294,7 +294,11 @@ uint64_t trt_frame_id(const trt_context* ctx);
 /* Outputs (replace the accumulator / colour / albedo / normal buffers,
  * Tracer/Renderer/DeviceRenderer.h:92-95, FinalizeFrame, Tracer/CUDA/GPU/CudaUtilityKernels.h:27-44).
  * Each writes width*height*4 floats; accum = (rgb sum, sample count), color = mean,
- * albedo = first-hit albedo, normal = (first-hit shading normal xyz, view depth). */
+ * albedo = (first-hit albedo rgb, 1), normal = (first-hit shading normal xyz, view depth).
+ * Albedo and normal are means over the pixel's accumulated samples (the reference keeps
+ * whichever sample wrote last, PathTracing.h:48-70); the normal mean is not renormalised.
+ * A camera ray that misses contributes the sky's radiance as albedo and a zero normal.
+ * trt_write_accum restarts these means with the next frame. */
 trt_status trt_read_accum(trt_context* ctx, float* rgba);
 trt_status trt_read_color(trt_context* ctx, float* rgba);
 trt_status trt_read_albedo(trt_context* ctx, float* rgba);

@@ -578,7 +578,8 @@ void orc_trace_any(void* p, const float* rays, uint32_t n, uint8_t* occ, int mod
 //       strategy (0 MIS, 1 light sampling only, 2 BSDF sampling only)
 // Outputs (nullable): path_rgb [W*H*spp*3] float, image_f32 [W*H*3] (float sum in
 // sample order / spp, as the device Finalize), image_f64 [W*H*3] (FP64 mean),
-// aov [W*H*8] (albedo rgb, pad, normal xyz, view depth) from sample 0,
+// aov [W*H*8] (albedo rgb, 1, normal xyz, view depth): per pixel means over the frame's
+// samples of the first-hit AOVs (float sums in sample order x 1/spp, as the device Finalize),
 // counts [2*max_depth] (extension rays, shadow rays per depth).
 // With a crop, only the paths of the crop's pixels are traced, with the keys and camera
 // rays of the whole film (pixel = y * W + x, path = sample * W * H + pixel); the outputs
@@ -611,6 +612,7 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
   const uint64_t npix = uint64_t(cw) * ch;  // pixels of the output (the crop)
   const uint64_t n = npix * rp.spp;
   std::vector<float> L(n * 3, 0.f);
+  std::vector<float> A(aov ? n * 8 : 0, 0.f);  // per path first-hit AOVs (zero for dead camera rays)
   const uint32_t D = rp.max_depth;
   std::vector<std::atomic<uint64_t>> cnt(size_t(2 * D));
   for (auto& c : cnt) c = 0;
@@ -618,7 +620,6 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
     const uint64_t smp = pi / npix, opix = pi % npix;  // sample, output pixel
     const uint64_t film_pixel = uint64_t(cy0 + opix / cw) * rp.width + cx0 + opix % cw;
     const uint32_t path = uint32_t(smp * film_pix + film_pixel);
-    const bool first_sample = path < film_pix;
     PathState ps;
     V3 acc = v3(0.f);
     std::vector<uint64_t> local(2 * D, 0);
@@ -631,10 +632,10 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
         ShadeOut out;
         shade_mode(s->view, s->samp.data(), rp, ps, h, d, out);
         if (!is_black(out.L)) acc = acc + out.L;
-        if (d == 0 && first_sample && aov) {
-          float* a = aov + size_t(opix) * 8;
-          a[0] = out.albedo.x; a[1] = out.albedo.y; a[2] = out.albedo.z; a[3] = 1.f;
-          a[4] = out.normal.x; a[5] = out.normal.y; a[6] = out.normal.z; a[7] = out.vdepth;
+        if (d == 0 && aov) {
+          float* a = &A[pi * 8];
+          a[0] = out.albedo.x; a[1] = out.albedo.y; a[2] = out.albedo.z; a[3] = out.vdepth;
+          a[4] = out.normal.x; a[5] = out.normal.y; a[6] = out.normal.z; a[7] = 0.f;
         }
         if (out.has_shadow) {
           local[D + d]++;
@@ -644,8 +645,6 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
         if (!out.has_ext) break;
         ps = out.next;
       }
-    } else if (first_sample && aov) {
-      std::memset(aov + size_t(opix) * 8, 0, 32);
     }
     L[pi * 3 + 0] = acc.x;
     L[pi * 3 + 1] = acc.y;
@@ -670,6 +669,14 @@ void orc_render(void* p, const float* cam, const uint32_t* uopts, const float* f
     for (int k = 0; k < 3; ++k) {
       if (image_f32) image_f32[px * 3 + k] = sf[k] * invs;
       if (image_f64) image_f64[px * 3 + k] = sd[k] / double(rp.spp);
+    }
+    if (aov) {
+      float sa[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      for (uint32_t smp = 0; smp < rp.spp; ++smp)
+        for (int k = 0; k < 8; ++k) sa[k] += A[(uint64_t(smp) * npix + px) * 8 + k];
+      float* a = aov + px * 8;
+      a[0] = sa[0] * invs; a[1] = sa[1] * invs; a[2] = sa[2] * invs; a[3] = 1.f;
+      a[4] = sa[4] * invs; a[5] = sa[5] * invs; a[6] = sa[6] * invs; a[7] = sa[3] * invs;
     }
   }
 }

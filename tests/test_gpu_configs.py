@@ -115,8 +115,8 @@ def test_config1_bench_frame(rend, soup_full):
     of ~0.01) and its 64 emitters lie inside it, so no light reaches the camera and every
     path's radiance is zero, on the GPU and in the oracle alike; the frame is checked
     through its first hits instead: the albedo, normal and view-depth AOVs of all 2M
-    pixels against the oracle's (sample 0). The counting pass over the same frame reports
-    the traversal stack's spills."""
+    pixels (means over the 8 samples) against the oracle's. The counting pass over the
+    same frame reports the traversal stack's spills."""
     sc, orc = soup_full
     rend.UpdateScene(sc, presplit=6.0)
     st = rend.RenderFrame()
@@ -125,7 +125,7 @@ def test_config1_bench_frame(rend, soup_full):
     assert st["overflow"] == 0 and st["paths"] == W * H * spp
     print("config-1 frame:", {k: st[k] for k in ("extend_rays", "shadow_rays", "frame_ms")})
     assert not rend.path_radiance().any()
-    ref = orc.render(render=dict(spp=1, max_depth=1))  # sample 0's first hits
+    ref = orc.render(render=dict(max_depth=1))  # every sample's first hit
     np.testing.assert_array_equal(rend.albedo()[..., :3], ref["aov"][..., 0:3])
     np.testing.assert_array_equal(rend.normal(), ref["aov"][..., 4:8])
     assert (ref["aov"][..., 0:3] > 0).any(axis=2).mean() > 0.5

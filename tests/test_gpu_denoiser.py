@@ -177,11 +177,12 @@ def test_denoising_policy_and_blend(rend):
     assert rend.DenoiseFrame() is None  # disabled
     rend.SetDenoiser(True, 4)
     ran = []
-    for _ in range(13):
+    for _ in range(12):  # sample counts 2 .. 13
         rend.RenderFrame()
         if rend.DenoiseFrame() is not None:
             ran.append(rend.SampleCount())
     assert ran == [4, 6, 9, 13], ran
+    assert rend.SampleCount() == 13
     # blend: against the unblended network output of the same frame (13 samples)
     blended = rend.denoised()[..., :3].astype(np.float64)
     spp = rend.SampleCount()

@@ -231,6 +231,26 @@ def test_path_radiance_cornell(rend):
     np.testing.assert_array_equal(rend.normal(), aov[..., 4:8])
 
 
+def test_aov_means_accumulate_over_frames(rend):
+    """Albedo / normal / view depth are per-pixel means over every accumulated sample:
+    two frames of 3 samples equal the oracle's 6-sample frame (up to the sums'
+    association: frame sums are added, the oracle sums the 6 in order); Reset restarts them."""
+    sc = scenes.cornell(width=40, height=32, spp=3, max_depth=2, seed=13)
+    sc["camera"] = dict(sc["camera"], aperture=0.05, focal_dist=1.6)  # samples differ in their first hits
+    rend.UpdateScene(sc)
+    rend.RenderFrame()
+    one = _oracle(sc).render()
+    np.testing.assert_array_equal(rend.normal(), one["aov"][..., 4:8])
+    rend.RenderFrame()
+    two = _oracle(sc).render(render=dict(spp=6))
+    np.testing.assert_allclose(rend.albedo()[..., :3], two["aov"][..., 0:3], rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(rend.normal(), two["aov"][..., 4:8], rtol=1e-6, atol=1e-6)
+    assert not np.allclose(one["aov"][..., 4:8], two["aov"][..., 4:8])  # the second frame changed them
+    rend.Reset()
+    rend.RenderFrame()
+    np.testing.assert_array_equal(rend.normal(), one["aov"][..., 4:8])
+
+
 def test_path_radiance_cornell_rr_deep(rend):
     sc = scenes.cornell(width=32, height=32, spp=8, max_depth=10, rr_depth=1, seed=9)
     rend.UpdateScene(sc)

@@ -106,6 +106,7 @@ struct trt_context {
   CameraParams cam{};
   uint32_t width = 0, height = 0;
   uint32_t sample_count = 0;
+  uint32_t aov_count = 0;  // samples in the AOV sums (restarted with the accumulator and by trt_write_accum)
   uint64_t last_paths = 0;
 
   // frame buffers
@@ -114,6 +115,8 @@ struct trt_context {
   DevBuf s_o, s_d, s_c;
   DevBuf radiance;
   DevBuf accum, color, albedo, normal;
+  DevBuf aov_sum_a, aov_sum_n;    // per pixel AOV sums
+  DevBuf path_aov_a, path_aov_n;  // per path first-hit AOVs of the current frame
   DevBuf tonemapped;  // RGBA8 display image
   DevBuf counters;
   DevBuf scratch_rays, scratch_hits, scratch_inst, scratch_occ;
@@ -159,6 +162,7 @@ void invalidate_frame(trt_context* c) { c->has_frame = false; }
 // and with it the denoiser's record of the sample count it last ran at (Denoiser::Reset).
 void reset_accumulation(trt_context* c) {
   c->sample_count = 0;
+  c->aov_count = 0;
   c->dn_last_spp = 0;
 }
 
@@ -276,6 +280,8 @@ trt_status ensure_frame(trt_context* ctx, uint32_t W, uint32_t H, uint32_t spp) 
     HIPC(ctx->color.ensure(npix * 16), "alloc color");
     HIPC(ctx->albedo.ensure(npix * 16), "alloc albedo");
     HIPC(ctx->normal.ensure(npix * 16), "alloc normal");
+    HIPC(ctx->aov_sum_a.ensure(npix * 16), "alloc AOV sums");
+    HIPC(ctx->aov_sum_n.ensure(npix * 16), "alloc AOV sums");
     HIPC(hipMemsetAsync(ctx->albedo.p, 0, npix * 16, ctx->stream), "clear albedo");
     HIPC(hipMemsetAsync(ctx->normal.p, 0, npix * 16, ctx->stream), "clear normal");
     ctx->width = W;
@@ -301,6 +307,8 @@ trt_status ensure_frame(trt_context* ctx, uint32_t W, uint32_t H, uint32_t spp) 
   }
   if (n > ctx->path_capacity) {
     HIPC(ctx->radiance.ensure(n * 16), "alloc radiance");
+    HIPC(ctx->path_aov_a.ensure(n * 16), "alloc path AOVs");
+    HIPC(ctx->path_aov_n.ensure(n * 16), "alloc path AOVs");
     ctx->path_capacity = n;
   }
   return TRT_OK;
@@ -377,6 +385,10 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
   fb.color = ctx->color.as<float4>();
   fb.albedo = ctx->albedo.as<float4>();
   fb.normal = ctx->normal.as<float4>();
+  fb.path_aov_a = ctx->path_aov_a.as<float4>();
+  fb.path_aov_n = ctx->path_aov_n.as<float4>();
+  fb.aov_sum_a = ctx->aov_sum_a.as<float4>();
+  fb.aov_sum_n = ctx->aov_sum_n.as<float4>();
   RayQueue q[2];
   for (int i = 0; i < 2; ++i) {
     q[i].o_tmin = ctx->q_o[i].as<float4>();
@@ -415,7 +427,7 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
 
   launch_zero_counters(ctr, stm);
   HIPC(hipEventRecord(ev[0], stm), "event");
-  launch_raygen(rp, n, cap, q[0], fb.radiance, ctr, stm);
+  launch_raygen(rp, n, cap, q[0], fb, ctr, stm);
   HIPC(hipEventRecord(ev[1], stm), "event");
   for (uint32_t d = 0; d < D; ++d) {
     const int cur = int(d & 1u), nxt = cur ^ 1;
@@ -448,7 +460,9 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
   }
   if (D > 0) HIPC(hipStreamWaitEvent(stm, ev2[2 * (D - 1) + 1], 0), "wait");
   HIPC(hipEventRecord(ev[3], stm), "event");
-  if (accumulate) launch_finalize(rp, fb.radiance, fb, ctx->sample_count == 0 ? 1u : 0u, stm);
+  if (accumulate)
+    launch_finalize(rp, fb.radiance, fb, ctx->sample_count == 0 ? 1u : 0u, ctx->aov_count == 0 ? 1u : 0u,
+                    ctx->aov_count + o.spp, stm);
   HIPC(hipEventRecord(ev[2], stm), "event");
   launch_sum_counts(ctr, cap, stm);
   HIPC(hipGetLastError(), "kernel launch");
@@ -461,7 +475,10 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
   ++ctx->frame_id;
   ctx->last_opts = o;
   ctx->last_sample_offset = rp.sample_offset;
-  if (accumulate) ctx->sample_count += o.spp;
+  if (accumulate) {
+    ctx->sample_count += o.spp;
+    ctx->aov_count += o.spp;
+  }
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->paths = n;
@@ -967,6 +984,8 @@ trt_status trt_write_accum(trt_context* ctx, uint32_t width, uint32_t height, co
     HIPC(ctx->color.ensure(npix * 16), "alloc color");
     HIPC(ctx->albedo.ensure(npix * 16), "alloc albedo");
     HIPC(ctx->normal.ensure(npix * 16), "alloc normal");
+    HIPC(ctx->aov_sum_a.ensure(npix * 16), "alloc AOV sums");
+    HIPC(ctx->aov_sum_n.ensure(npix * 16), "alloc AOV sums");
     HIPC(hipMemsetAsync(ctx->albedo.p, 0, npix * 16, ctx->stream), "clear albedo");
     HIPC(hipMemsetAsync(ctx->normal.p, 0, npix * 16, ctx->stream), "clear normal");
     ctx->width = width;
@@ -982,6 +1001,7 @@ trt_status trt_write_accum(trt_context* ctx, uint32_t width, uint32_t height, co
   HIPC(hipMemcpyAsync(ctx->color.p, col.data(), npix * 16, hipMemcpyHostToDevice, ctx->stream), "upload color");
   HIPC(hipStreamSynchronize(ctx->stream), "upload accum");
   ctx->sample_count = uint32_t(n);
+  ctx->aov_count = 0;  // the AOV sums restart with the next frame
   ctx->dn_last_spp = 0;
   invalidate_frame(ctx);
   return TRT_OK;

@@ -104,8 +104,12 @@ struct FrameBuffers {
   float4* radiance;   // per path (rgb)
   float4* accum;      // per pixel: rgb sum, sample count
   float4* color;      // per pixel mean
-  float4* albedo;     // per pixel first-hit albedo (sample 0)
-  float4* normal;     // per pixel first-hit shading normal (xyz), view depth (w)
+  float4* albedo;     // per pixel mean over the accumulated samples of the first-hit albedo
+  float4* normal;     // per pixel mean of the first-hit shading normal (xyz) and view depth (w)
+  float4* path_aov_a; // per path: first-hit albedo (rgb), view depth (w)
+  float4* path_aov_n; // per path: first-hit shading normal (xyz)
+  float4* aov_sum_a;  // per pixel sums of the above over the accumulated samples
+  float4* aov_sum_n;
 };
 
 // Adjoint (path replay) state. Records: one per (path, depth) slot r = path*depth + d,
@@ -131,7 +135,7 @@ int trace_blocks_per_cu(const DevAccel& acc);
 // Kernel launchers (kernels.hip). All run on `stream`. `cap` is the per-segment
 // capacity of every queue; out_cnt / shadow_cnt point at kSegs counter lines.
 void launch_zero_counters(uint32_t* ctr, hipStream_t stream);
-void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, float4* radiance,
+void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, const FrameBuffers& fb,
                    uint32_t* ctr, hipStream_t stream);
 void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr, int grid, bool stats,
                    hipStream_t stream);
@@ -144,8 +148,10 @@ void launch_sum_counts(uint32_t* ctr, uint32_t cap, hipStream_t stream);
 // Adds the set bits of a touch bitmap (`words` 32-bit words) to *out and clears it.
 void launch_touch_count(uint32_t* bits, uint32_t words, unsigned long long* out, hipStream_t stream);
 void launch_tonemap(const float4* src, uint32_t n, const TonemapParams& tp, uchar4* dst, hipStream_t stream);
+// reset: restart the accumulator; aov_reset: restart the AOV sums; aov_count: samples in
+// the AOV sums after this frame.
 void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
-                     hipStream_t stream);
+                     uint32_t aov_reset, uint32_t aov_count, hipStream_t stream);
 void launch_raygen_adj(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, const float4* radiance,
                        const AdjBuffers& ab, uint32_t* ctr, hipStream_t stream);
 void launch_shade_adj(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,

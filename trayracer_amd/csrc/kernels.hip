@@ -1010,13 +1010,17 @@ __device__ __forceinline__ void store_primary(const RayQueue& q, uint32_t s, uin
 }
 
 __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams rp, uint32_t n_paths, uint32_t per, uint32_t cap,
-                                                    RayQueue q, float4* radiance, uint32_t* ctr) {
+                                                    RayQueue q, FrameBuffers fb, uint32_t* ctr) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   primary_counts(n_paths, per, ctr + seg_ctr(kSegExt, 0));
   if (p < n_paths) {
     PathState ps;
     const bool alive = raygen(rp, p, ps);
-    radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    fb.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!alive) {  // Shade never sees this path: its first-hit AOVs are zero
+      fb.path_aov_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+      fb.path_aov_n[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     store_primary(q, primary_slot(p, per, cap), p, ps, alive, rp.eps);
   }
 }
@@ -1210,10 +1214,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       const bool work = live && (keep_misses || __float_as_uint(hh.w) != kNoHit);
       if (live && !work && depth == 0) {
         const uint32_t path = q.meta[slot] >> 2;
-        if (path < npix) {
-          fb.albedo[path] = make_float4(0.f, 0.f, 0.f, 1.f);
-          fb.normal[path] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        fb.path_aov_a[path] = make_float4(0.f, 0.f, 0.f, 0.f);
+        fb.path_aov_n[path] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
       const uint64_t m = __ballot(work);
       if (work) {
@@ -1261,9 +1263,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         Lr.z += out.L.z;
         fb.radiance[path] = Lr;
       }
-      if (depth == 0 && path < npix) {
-        fb.albedo[path] = make_float4(out.albedo.x, out.albedo.y, out.albedo.z, 1.f);
-        fb.normal[path] = make_float4(out.normal.x, out.normal.y, out.normal.z, out.vdepth);
+      if (depth == 0) {
+        fb.path_aov_a[path] = make_float4(out.albedo.x, out.albedo.y, out.albedo.z, out.vdepth);
+        fb.path_aov_n[path] = make_float4(out.normal.x, out.normal.y, out.normal.z, 0.f);
       }
     }
     // move the unprocessed tail (at most 63 slots, all at or above `take`) to the front
@@ -1295,17 +1297,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   }
 }
 
+__device__ __forceinline__ float4 add4(float4 a, float4 b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+// Per pixel, in sample order: the frame's radiance into the accumulator (mean colour),
+// and its first-hit AOVs into their sums (means over the accumulated samples: the
+// denoiser's guide images, anti-aliased like the colour).
 __global__ __launch_bounds__(kBlock) void k_finalize(RenderParams rp, const float4* radiance, FrameBuffers fb,
-                                                      uint32_t reset) {
+                                                      uint32_t reset, uint32_t aov_reset, uint32_t aov_count) {
   const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t npix = rp.width * rp.height;
   if (pix >= npix) return;
   float sx = 0.f, sy = 0.f, sz = 0.f;
+  float4 sa = make_float4(0.f, 0.f, 0.f, 0.f), sn = make_float4(0.f, 0.f, 0.f, 0.f);
   for (uint32_t s = 0; s < rp.spp; ++s) {
-    const float4 L = radiance[static_cast<size_t>(s) * npix + pix];
+    const size_t i = static_cast<size_t>(s) * npix + pix;
+    const float4 L = radiance[i];
     sx += L.x;
     sy += L.y;
     sz += L.z;
+    sa = add4(sa, fb.path_aov_a[i]);
+    sn = add4(sn, fb.path_aov_n[i]);
   }
   float4 acc = reset ? make_float4(0.f, 0.f, 0.f, 0.f) : fb.accum[pix];
   acc.x += sx;
@@ -1315,6 +1328,15 @@ __global__ __launch_bounds__(kBlock) void k_finalize(RenderParams rp, const floa
   fb.accum[pix] = acc;
   const float inv = acc.w > 0.f ? 1.f / acc.w : 0.f;
   fb.color[pix] = make_float4(acc.x * inv, acc.y * inv, acc.z * inv, 1.f);
+  if (!aov_reset) {
+    sa = add4(fb.aov_sum_a[pix], sa);
+    sn = add4(fb.aov_sum_n[pix], sn);
+  }
+  fb.aov_sum_a[pix] = sa;
+  fb.aov_sum_n[pix] = sn;
+  const float ia = 1.f / static_cast<float>(aov_count);
+  fb.albedo[pix] = make_float4(sa.x * ia, sa.y * ia, sa.z * ia, 1.f);
+  fb.normal[pix] = make_float4(sn.x * ia, sn.y * ia, sn.z * ia, sa.w * ia);
 }
 
 __global__ __launch_bounds__(kBlock) void k_tonemap(const float4* src, uint32_t n, TonemapParams tp,
@@ -1594,11 +1616,11 @@ void launch_touch_count(uint32_t* bits, uint32_t words, unsigned long long* out,
                      words, out);
 }
 
-void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, float4* radiance,
+void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQueue q, const FrameBuffers& fb,
                    uint32_t* ctr, hipStream_t stream) {
   const uint32_t blocks = (n_paths + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_raygen, dim3(blocks), dim3(kBlock), 0, stream, rp, n_paths, prim_per_seg(n_paths), cap, q,
-                     radiance, ctr);
+                     fb, ctr);
 }
 
 void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr, int grid, bool stats,
@@ -1648,10 +1670,10 @@ void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn 
 }
 
 void launch_finalize(const RenderParams& rp, const float4* radiance, FrameBuffers fb, uint32_t reset,
-                     hipStream_t stream) {
+                     uint32_t aov_reset, uint32_t aov_count, hipStream_t stream) {
   const uint32_t npix = rp.width * rp.height;
   hipLaunchKernelGGL(k_finalize, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, rp, radiance, fb,
-                     reset);
+                     reset, aov_reset, aov_count);
 }
 
 
