@@ -426,17 +426,44 @@ bool carry_byte(float v, uint32_t byte, bool upper, float& out) {
 
 }  // namespace
 
+namespace {
+
+// Smallest float not below v.
+float float_up(double v) {
+  float f = float(v);
+  if (double(f) < v) f = std::nextafter(f, HUGE_VALF);
+  return f;
+}
+
+// Centre and half extent of [lo, hi] as stored (see DevNode48); cb / hb: the low bytes the
+// centre and the half extent carry, or -1 for none.
+bool centre_half(float lo, float hi, int cb, int hb, float& c, float& h) {
+  if (!std::isfinite(lo) || !std::isfinite(hi)) return false;
+  c = float(0.5 * (double(lo) + double(hi)));
+  if (cb >= 0 && !carry_byte(c, uint32_t(cb), false, c)) return false;
+  const double half = std::max(double(hi) - double(c), double(c) - double(lo));
+  h = float_up(float_up(half) * (1.0 + kHalfExtentMargin));
+  if (!std::isfinite(h)) return false;
+  if (hb >= 0 && !carry_byte(h, uint32_t(hb), true, h)) return false;
+  return true;
+}
+
+}  // namespace
+
 bool pack_nodes(const std::vector<DevNode2>& in, std::vector<DevNode48>& out) {
   out.resize(in.size());
   for (size_t i = 0; i < in.size(); ++i) {
     const DevNode2& n = in[i];
     DevNode48& d = out[i];
-    for (int k = 0; k < 4; ++k) {
-      d.r2[k] = n.r2[k];
-      // row layout: lo.x, hi.x, lo.y, hi.y -> odd entries are upper bounds
-      const bool upper = (k & 1) != 0;
-      const uint32_t b0 = (uint32_t(n.c[0]) >> (8 * k)) & 0xffu, b1 = (uint32_t(n.c[1]) >> (8 * k)) & 0xffu;
-      if (!carry_byte(n.r0[k], b0, upper, d.r0[k]) || !carry_byte(n.r1[k], b1, upper, d.r1[k])) return false;
+    const float* xy[2] = {n.r0, n.r1};  // lo.x, hi.x, lo.y, hi.y per child
+    float* row[2] = {d.r0, d.r1};
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t ref = uint32_t(n.c[c]);
+      for (int k = 0; k < 2; ++k) {
+        const int cb = int((ref >> (16 * k)) & 0xffu), hb = int((ref >> (16 * k + 8)) & 0xffu);
+        if (!centre_half(xy[c][2 * k], xy[c][2 * k + 1], cb, hb, row[c][2 * k], row[c][2 * k + 1])) return false;
+      }
+      if (!centre_half(n.r2[2 * c], n.r2[2 * c + 1], -1, -1, d.r2[2 * c], d.r2[2 * c + 1])) return false;
     }
   }
   return true;

@@ -67,25 +67,37 @@ struct alignas(16) DevNode2 {
 
 inline int32_t leaf_code(uint32_t first, uint32_t count) { return ~static_cast<int32_t>((first << 3) | count); }
 
-// Traversal layout on the device: 48 bytes, DevNode2's three box rows with the two
-// child references carried in the low byte of the x and y bounds (child0 in row0's four
+// Traversal layout on the device: 48 bytes, each child box as a centre and a half
+// extent per axis,
+//   row0 = (c0.cx, c0.hx, c0.cy, c0.hy), row1 = (c1.cx, c1.hx, c1.cy, c1.hy),
+//   row2 = (c0.cz, c0.hz, c1.cz, c1.hz),
+// so that a slab's near and far distances are ct - |ht| and ct + |ht| with
+// ct = (c - o) * inv and ht = h * inv: floating-point adds and multiplies only, with no
+// per-axis min / max (which issue at half the rate of FP32 arithmetic on gfx950). The two
+// child references ride in the low bytes of the x and y values (child0 in row0's four
 // floats, child1 in row1's, least significant byte first), so a node is three 16-byte
-// loads instead of four. Each bound that carries a byte is moved outward (a lower bound
-// down, an upper bound up) to the nearest float whose low byte is that byte, at most
-// 256 ulp, so every stored box contains the exact one and the slab test stays
+// loads. A centre that carries a byte is moved down to the nearest float with that low
+// byte (at most 255 ulp); the half extent then covers the exact box from the stored
+// centre on both sides, is enlarged by 2^-10 of itself (which covers the rounding of
+// ct - |ht| and ct + |ht| beyond the relative error the traversal's far-distance slack
+// covers) and, if it carries a byte, moved up to the nearest float with that low byte.
+// Every stored box therefore contains the exact one and the slab test stays
 // conservative.
 struct alignas(16) DevNode48 {
   float r0[4], r1[4], r2[4];
 };
 
-// Child reference carried by the low bytes of one row (x0, y0, x1, y1 bounds).
+// Child reference carried by the low bytes of one row (cx, hx, cy, hy).
 TRT_FI int32_t node_child_ref(float a, float b, float c, float d) {
   return static_cast<int32_t>((f2u(a) & 0xffu) | ((f2u(b) & 0xffu) << 8) | ((f2u(c) & 0xffu) << 16) |
                               (f2u(d) << 24));
 }
 
-// Converts nodes to the 48-byte layout. False if a bound cannot carry its byte (an
-// infinite or maximal bound).
+// Relative enlargement of every stored half extent.
+constexpr double kHalfExtentMargin = 1.0 / 1024.0;
+
+// Converts nodes to the 48-byte layout. False for a non-finite bound or one that cannot
+// carry its byte.
 bool pack_nodes(const std::vector<DevNode2>& in, std::vector<DevNode48>& out);
 
 // Converts a build tree into the device layout (depth-first order, node 0 = root).

@@ -150,51 +150,63 @@ __device__ __forceinline__ void touch(uint32_t* bits, uint32_t i) {
   if (bits) atomicOr(&bits[i >> 5], 1u << (i & 31u));
 }
 
-// Ray/box slab test of the device traversals (Ize 2013's conservative form): each
-// distance is (plane - o) * inv, within 2^-23 relative of exact, and the far distance
-// is widened by kBoxSlack, so no box the ray touches is culled and traversal equals a
-// brute-force loop. inv = 1/d is the IEEE reciprocal: a zero (or denormal) component
-// gives +-inf, and a ray parallel to an axis then gets (-inf, +inf) inside the slab,
-// an empty interval outside it and, lying exactly on a bounding plane, 0 * inf = NaN.
-// Per axis, the entry and exit are the NaN-propagating minimum and maximum of the two
-// distances (v_minimum3/v_maximum3), and the axes are combined with max/min that ignore
-// NaN (v_max3/v_min3): an on-plane parallel axis limits neither end, as its true
-// unbounded interval would not. (Clamping the reciprocal to a large finite value instead
-// turns the on-plane case into a zero exit distance and culls the box.)
+// Ray/box slab test of the device traversals (after Ize 2013's conservative form), on
+// boxes stored as centre c and half extent h per axis. With x = c - o and s = sign(inv)
+// (per ray, +-1), a slab's entry and exit are (x - s h) * inv and (x + s h) * inv, the
+// entry never above the exit; each is within a few ulp of itself plus u |x inv|. The
+// stored half extent exceeds the exact box by 2^-10 of itself, which covers the error
+// relative to h, and the far distance is widened by kBoxSlack, which covers the rest, so
+// no box the ray touches is culled and traversal equals a brute-force loop. inv = 1/d is
+// the IEEE reciprocal: for a zero direction component it is +-inf and the products give
+// the exact slab of a ray parallel to the axis: (-inf, +inf) when the origin lies inside
+// it, both +inf or both -inf (an empty interval) when outside, and NaN for an end whose
+// plane the origin lies on; the axes are combined with max / min that ignore NaN, so that
+// end limits nothing. Only FP32 subtracts, multiplies and adds, then three-operand max /
+// min: on gfx950 FP32 arithmetic issues at twice the rate of min / max, so this form is
+// cheaper than ordering each axis's two plane distances with min / max.
 struct RayBox {
   V3 o;
   V3 inv;  // 1 / d, infinite for a zero component
+  V3 sgn;  // +-1, the sign of inv
 };
 
 __device__ __forceinline__ RayBox ray_box_setup(V3 o, V3 d) {
   RayBox b;
   b.o = o;
   b.inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  b.sgn = v3(__builtin_copysignf(1.f, b.inv.x), __builtin_copysignf(1.f, b.inv.y), __builtin_copysignf(1.f, b.inv.z));
   return b;
 }
 
-__device__ __forceinline__ bool slab_hw(float lox, float hix, float loy, float hiy, float loz, float hiz,
+__device__ __forceinline__ bool slab_ch(float cx, float hx, float cy, float hy, float cz, float hz,
                                         const RayBox& b, float tmin, float tmax, float& tnear) {
-  const float tx0 = (lox - b.o.x) * b.inv.x, tx1 = (hix - b.o.x) * b.inv.x;
-  const float ty0 = (loy - b.o.y) * b.inv.y, ty1 = (hiy - b.o.y) * b.inv.y;
-  const float tz0 = (loz - b.o.z) * b.inv.z, tz1 = (hiz - b.o.z) * b.inv.z;
-  const float nx = __builtin_elementwise_minimum(tx0, tx1), fx = __builtin_elementwise_maximum(tx0, tx1);
-  const float ny = __builtin_elementwise_minimum(ty0, ty1), fy = __builtin_elementwise_maximum(ty0, ty1);
-  const float nz = __builtin_elementwise_minimum(tz0, tz1), fz = __builtin_elementwise_maximum(tz0, tz1);
-  const float n = __builtin_fmaxf(__builtin_fmaxf(nx, ny), __builtin_fmaxf(nz, tmin));
-  const float f = __builtin_fminf(__builtin_fminf(fx, fy), __builtin_fminf(fz, tmax));
+  const float xx = cx - b.o.x, sx = hx * b.sgn.x;
+  const float xy = cy - b.o.y, sy = hy * b.sgn.y;
+  const float xz = cz - b.o.z, sz = hz * b.sgn.z;
+  const float n = __builtin_fmaxf(__builtin_fmaxf((xx - sx) * b.inv.x, (xy - sy) * b.inv.y),
+                                  __builtin_fmaxf((xz - sz) * b.inv.z, tmin));
+  const float f = __builtin_fminf(__builtin_fminf((xx + sx) * b.inv.x, (xy + sy) * b.inv.y),
+                                  __builtin_fminf((xz + sz) * b.inv.z, tmax));
   tnear = n;
   return n <= f * kBoxSlack;
 }
 
-// One node: three 16-byte loads, child references from the bounds' low bytes.
+// Child reference from the low bytes of one node row (v_perm_b32 gathers two bytes per
+// instruction).
+__device__ __forceinline__ int32_t row_child_ref(float4 r) {
+  const uint32_t lo = __builtin_amdgcn_perm(__float_as_uint(r.y), __float_as_uint(r.x), 0x0c0c0400u);
+  const uint32_t hi = __builtin_amdgcn_perm(__float_as_uint(r.w), __float_as_uint(r.z), 0x04000c0cu);
+  return static_cast<int32_t>(lo | hi);
+}
+
+// One node: three 16-byte loads, child references from the x / y values' low bytes.
 __device__ __forceinline__ void load_node(const DevNode48* __restrict__ nodes, int32_t i, float4& r0, float4& r1,
                                           float4& r2, int4& c) {
   const float4* p = reinterpret_cast<const float4*>(nodes + i);
   r0 = p[0];
   r1 = p[1];
   r2 = p[2];
-  c = make_int4(node_child_ref(r0.x, r0.y, r0.z, r0.w), node_child_ref(r1.x, r1.y, r1.z, r1.w), 0, 0);
+  c = make_int4(row_child_ref(r0), row_child_ref(r1), 0, 0);
 }
 
 // BVH2 traversal of one bottom-level tree (or the single-level scene) for an
@@ -215,8 +227,8 @@ __device__ __forceinline__ bool traverse_bvh(const DevNode48* __restrict__ nodes
       load_node(nodes, node, r0, r1, r2, c);
       if (STATS) r.nodes++;
       float t0, t1;
-      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
-      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+      const bool h0 = slab_ch(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+      const bool h1 = slab_ch(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
       if (h0 && h1) {
         int32_t nearc = c.x, farc = c.y;
         if (t1 < t0) {
@@ -320,8 +332,8 @@ __device__ __forceinline__ bool traverse_ww(const DevNode48* __restrict__ nodes,
       load_node(nodes, node, r0, r1, r2, c);
       if (STATS) r.nodes++;
       float t0, t1;
-      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
-      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+      const bool h0 = slab_ch(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+      const bool h1 = slab_ch(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
       if (h0 && h1) {
         int32_t nearc = c.x, farc = c.y;
         if (t1 < t0) {
@@ -642,8 +654,8 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
           touch(acc.touch_nodes, static_cast<uint32_t>(node));
         }
         float t0, t1;
-        const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
-        const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+        const bool h0 = slab_ch(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+        const bool h1 = slab_ch(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
         if (h0 && h1) {
           int32_t nearc = c.x, farc = c.y;
           if (t1 < t0) {
@@ -830,8 +842,8 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
           if (in_blas) touch(acc.touch_nodes, static_cast<uint32_t>(node));
         }
         float t0, t1;
-        const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
-        const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+        const bool h0 = slab_ch(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+        const bool h1 = slab_ch(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
         if (h0 && h1) {
           int32_t nearc = c.x, farc = c.y;
           if (t1 < t0) {
@@ -911,8 +923,8 @@ __device__ __forceinline__ bool trace_ray(const DevAccel& acc, V3 o, V3 d, float
       load_node(acc.tlas, node, r0, r1, r2, c);
       if (STATS) r.nodes++;
       float t0, t1;
-      const bool h0 = slab_hw(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
-      const bool h1 = slab_hw(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
+      const bool h0 = slab_ch(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
+      const bool h1 = slab_ch(r1.x, r1.y, r1.z, r1.w, r2.z, r2.w, rb, tmin, r.t, t1);
       if (h0 && h1) {
         int32_t nearc = c.x, farc = c.y;
         if (t1 < t0) {
