@@ -812,6 +812,9 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
     ctx->sky.sun_power = sun_selection_power(ctx->sky, ctx->scene_radius);
   }
 
+  // the traversal addresses nodes by 32-bit byte offsets
+  if (nodes.size() >= (uint64_t(1) << 32) / sizeof(DevNode48) || tlas_nodes.size() >= (uint64_t(1) << 32) / sizeof(DevNode48))
+    return fail(ctx, TRT_UNSUPPORTED, "BVH too large: more than 89M nodes in one tree array");
   std::vector<DevNode48> nodes48, tlas48;
   if (!pack_nodes(nodes, nodes48) || !pack_nodes(tlas_nodes, tlas48))
     return fail(ctx, TRT_UNSUPPORTED, "scene bounds too large for the node layout (infinite coordinates)");

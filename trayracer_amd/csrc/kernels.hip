@@ -202,7 +202,11 @@ __device__ __forceinline__ int32_t row_child_ref(float4 r) {
 // One node: three 16-byte loads, child references from the x / y values' low bytes.
 __device__ __forceinline__ void load_node(const DevNode48* __restrict__ nodes, int32_t i, float4& r0, float4& r1,
                                           float4& r2, int4& c) {
-  const float4* p = reinterpret_cast<const float4*>(nodes + i);
+  // a 32-bit byte offset from the (wave-uniform) array base: 32-bit arithmetic and a load
+  // with a scalar base instead of 64-bit address arithmetic per visit (trt_build keeps
+  // node arrays below 4 GiB)
+  const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) +
+                                                    static_cast<uint32_t>(i) * 48u);
   r0 = p[0];
   r1 = p[1];
   r2 = p[2];
