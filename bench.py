@@ -32,7 +32,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-denoise", action="store_true",
                     help="skip the time-to-relMSE measurement (denoiser training + convergence runs)")
-    ap.add_argument("--denoise-steps", type=int, default=4000)
+    ap.add_argument("--denoise-steps", type=int, default=12000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--presplit", type=float, default=6.0,
                     help="BVH spatial pre-splitting budget, extra references per triangle (0 = off)")
@@ -182,9 +182,11 @@ def denoise_metric(rend, steps):
 
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     rend.denoiser_init(seed=0)
-    t_pairs = C.build_training_set(rend, views=(("soup", 4), ("inst", 4)), log=log)
-    t_train, losses = C.train(rend, steps=steps, batch=8, crop=128, lr=1e-3)
-    log(f"denoiser trained: {steps} steps in {t_train:.1f} s, loss {float(losses[-50:].mean()):.5f}")
+    t_pairs = C.build_training_set(rend, log=log)
+    scale = steps / sum(n for n, _ in C.TRAIN_SCHEDULE)
+    schedule = tuple((max(1, int(round(n * scale))), lr) for n, lr in C.TRAIN_SCHEDULE)
+    t_train, losses = C.train(rend, schedule=schedule, batch=8, crop=128)
+    log(f"denoiser trained: {len(losses)} steps in {t_train:.1f} s, loss {float(losses[-50:].mean()):.5f}")
     soup = C.time_to_relmse(rend, C.make_scene("soup"), C.SOUP_CAMERA, max_plain_spp=2048, max_denoise_spp=64, log=log)
     inst = C.time_to_relmse(rend, C.make_scene("inst"), C.INST_CAMERA, max_plain_spp=256, max_denoise_spp=64, log=log)
 
@@ -201,9 +203,10 @@ def denoise_metric(rend, steps):
                       "targets compare relMSE minus the reference's estimated own error",
             "soup": pick(soup, "1M-triangle soup under the sky, 1920x1080, camera (0.5, 0.5, -1.2), depth 8"),
             "instanced": pick(inst, "1000-instance scene, 1920x1080, camera (50, 50, -20), depth 6"),
-            "train": {"pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1), "steps": steps, "batch": 8,
-                      "crop": 128, "views": "4 soup (sky) + 4 instanced random cameras, inputs 1/2/4/8 spp, "
-                                            "targets 1024 spp", "final_loss": float(losses[-50:].mean())},
+            "train": {"pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1), "steps": int(len(losses)),
+                      "schedule": [[n, lr] for n, lr in schedule], "batch": 8, "crop": 128,
+                      "views": f"{C.TRAIN_VIEWS} random cameras, inputs {C.TRAIN_INPUT_SPP} spp, targets 1024 spp",
+                      "final_loss": float(losses[-50:].mean())},
             "network_roofline": network_roofline(rend)}
 
 
