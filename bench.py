@@ -104,14 +104,15 @@ def host_cpu():
 CPU_CROP = (832, 412, 256, 256)  # the 1080p film's centre
 
 
-def cpu_baseline(threads, presplit, crop_spp=128):
+def cpu_baseline(threads, presplit, crop_spp=640):
     """The host build of the same per-vertex code (camera, traversal, BSDF, integrator)
     on the host cores, traversing the product's own binned-SAH tree over pre-split
     references (the GPU's tree, built by the same host code), near child first: the
     Cornell configuration (256x256, 16 spp, depth 4) in full, and the 1M-triangle soup's
     frame on a fixed 256x256 window at the film's centre, the same paths as the GPU
-    frame's. One spp of that window is only ~0.2 M rays, so `crop_spp` samples are
-    traced for a stable rate."""
+    frame's. One spp of that window (~0.24 M rays, a few tens of ms) is timed and
+    reported as well; the headline rate traces `crop_spp` samples, about 10 s of work on
+    16 host threads, for a stable figure."""
     from oracle import Oracle
     from trayracer_amd import scenes
     model, affinity, _ = host_cpu()
@@ -130,6 +131,10 @@ def cpu_baseline(threads, presplit, crop_spp=128):
     build_s = time.perf_counter() - t0
     x0, y0, w, h = CPU_CROP
     t0 = time.perf_counter()
+    one = orc.render(render=dict(spp=1), crop=CPU_CROP, threads=threads)
+    dt_one = time.perf_counter() - t0
+    rays_one = int(one["counts"].sum())
+    t0 = time.perf_counter()
     out1 = orc.render(render=dict(spp=crop_spp), crop=CPU_CROP, threads=threads)
     dt1 = time.perf_counter() - t0
     rays1 = int(out1["counts"].sum())
@@ -141,6 +146,7 @@ def cpu_baseline(threads, presplit, crop_spp=128):
                   f"({x0}, {y0}), {crop_spp} spp: {rays1} rays in {dt1:.2f} s on {threads} threads; the product's "
                   f"pre-split SAH tree (build {build_s:.1f} s, not timed)",
         "cpu_model": model, "affinity_cpus": affinity,
+        "crop_1spp": {"rays": rays_one, "seconds": round(dt_one, 4), "mrays_s": round(rays_one / dt_one / 1e6, 3)},
         "cornell_full": {"config": "256x256, 16 spp, depth 4, RR from depth 3", "seconds": round(dt0, 3),
                          "rays": rays0, "mrays_s": round(rays0 / dt0 / 1e6, 3)},
     }
@@ -288,6 +294,7 @@ def extend_roofline(tc, info, ext_rays, ext_ms, launches, workload):
                           "frac": round(loaded / vm_peak, 4) if vm_peak else None,
                           "bytes_per_ray": round(per_ray, 1), "nodes_per_ray": round(tc["closest_nodes_per_ray"], 2),
                           "tris_per_ray": round(tc["closest_tris_per_ray"], 2),
+                          "loads_16B_per_ray": round(per_ray / 16.0, 1),
                           "peak_source": "trt_measure_gather: independent random 48-byte gathers, 16 KiB buffer",
                           "gather_ceilings_gbs": ceil},
         "stack_spills_per_ray": round(tc["closest_spills"] / max(1, tc["closest_rays"]), 4),
