@@ -414,7 +414,8 @@ trt_status trt_denoiser_load(trt_context* ctx, const char* path);
 trt_status trt_denoiser_dataset(trt_context* ctx, uint32_t n, uint32_t width, uint32_t height);
 trt_status trt_denoiser_capture(trt_context* ctx, uint32_t slot, uint32_t kind);
 /* `steps` Adam steps on random crop x crop windows (crop % 128 == 0), batch crops per
- * step; losses (nullable) receives one mean squared log-space error per step. */
+ * step, on the mean absolute log-space error (L1); losses (nullable) receives that loss
+ * per step. */
 trt_status trt_denoiser_train(trt_context* ctx, uint32_t steps, uint32_t batch, uint32_t crop, float lr, uint32_t seed,
                               float* losses);
 /* Denoises the accumulated frame (needs width % 128 == 0, height % 8 == 0). */

@@ -113,7 +113,9 @@ def test_train_and_denoise_cornell(rend, tmp_path):
         for _ in range(4):
             rend.RenderFrame()
         rend.denoiser_capture(k, 1)
-    losses = rend.denoiser_train(120, batch=4, crop=128, lr=2e-3, seed=3)
+    # 300 steps: with the L1 loss the first ~150 steps barely change the held-out error
+    # (relMSE 0.93 after 120 steps, 0.068 after 300, 0.022 after 600)
+    losses = rend.denoiser_train(300, batch=4, crop=128, lr=2e-3, seed=3)
     print("loss first/last:", losses[:3], losses[-3:])
     assert np.isfinite(losses).all()
     assert losses[-10:].mean() < 0.5 * losses[:5].mean()
@@ -134,7 +136,7 @@ def test_train_and_denoise_cornell(rend, tmp_path):
     def relmse(x):
         return float(((x - ref) ** 2 / (ref ** 2 + 1e-2)).mean())
     print("relMSE noisy", relmse(noisy), "denoised", relmse(den), "denoise ms", st)
-    assert relmse(den) < 0.5 * relmse(noisy)
+    assert relmse(den) < 0.25 * relmse(noisy)
     # weights round-trip through the own file format
     w = rend.denoiser_params()
     path = tmp_path / "unet.trtdn"
