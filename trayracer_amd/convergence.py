@@ -84,7 +84,7 @@ def make_scene(kind, width=1920, height=1080):
     return scenes.cornell(width=width, height=height)
 
 
-TRAIN_VIEWS = (("soup", 6), ("inst", 2))
+TRAIN_VIEWS = (("soup", 8), ("inst", 2))
 TRAIN_INPUT_SPP = (1, 2, 4, 8, 16, 32, 64)
 TRAIN_SCHEDULE = ((6000, 1e-3), (3000, 3e-4), (3000, 1e-4))
 
@@ -134,7 +134,11 @@ def train(rend, schedule=TRAIN_SCHEDULE, batch=8, crop=128, seed=0):
 def time_to_relmse(rend, scene, camera, width=1920, height=1080, target=0.01, ref_spp=8192, max_plain_spp=1024,
                    max_denoise_spp=64, frame_spp=32, log=print):
     """Returns a dict with the plain and denoised times to `target` (seconds) and the
-    reference's estimated own error."""
+    reference's estimated own error. Plain: sample counts double until the target is
+    met, and the crossing is interpolated under relMSE ~ 1/spp. Denoised: sample counts
+    double until a denoised frame meets the target, then whole sample counts inside the
+    last doubling are bisected; the time is the measured one (render + inference) at
+    the smallest count found to meet it."""
     rend.UpdateScene(dict(scene, render=dict(scene["render"], width=width, height=height)))
     rend.SetCamera(camera)
     t_a = _render_spp(rend, ref_spp // 2, seed=999, frame_spp=frame_spp)
@@ -155,17 +159,31 @@ def time_to_relmse(rend, scene, camera, width=1920, height=1080, target=0.01, re
             break
         k *= 2
     den = []
-    k = 1
-    while k <= max_denoise_spp:
+
+    def denoised_at(k):
         t = _render_spp(rend, k, seed=17, frame_spp=frame_spp)
         t0 = time.perf_counter()
         st = rend.denoise()
         td = time.perf_counter() - t0
         e = relmse(rend.denoised(), ref)
         den.append((k, t + td, e, st["total_ms"]))
-        if e - ref_err <= target:
+        return e - ref_err <= target
+
+    k = 1
+    while k <= max_denoise_spp:
+        if denoised_at(k):
             break
         k *= 2
+    # the doubling brackets the crossing in (k / 2, k]: bisect over whole sample counts
+    lo, hi = k // 2, k
+    if k <= max_denoise_spp and k > 1:
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            if denoised_at(mid):
+                hi = mid
+            else:
+                lo = mid
+    passing = [d for d in den if d[2] - ref_err <= target]
     # plain: the sample count where the reference-corrected relMSE crosses the target
     # under the 1/spp law of Monte Carlo error from the last measured count (between the
     # last two doublings when it crossed, beyond the budget otherwise), times the
@@ -175,8 +193,8 @@ def time_to_relmse(rend, scene, camera, width=1920, height=1080, target=0.01, re
     n_star = kp * ep_true / target
     t_plain = tp / kp * n_star
     plain_how = "interpolated (relMSE ~ 1/spp)" if ep_true <= target else "extrapolated (relMSE ~ 1/spp)"
-    kd, td_, ed, _ = den[-1]
-    reached = ed - ref_err <= target
+    reached = bool(passing)
+    kd, td_, ed, _ = min(passing, key=lambda d: d[0]) if reached else den[-1]
     t_den = td_ if reached else None
     why = None
     if not reached:
