@@ -1,7 +1,8 @@
 // Property check of trt::pack_nodes, the 48-byte device node layout (centre and half
 // extent per axis; host code only). Prints "ok <nodes>" or the first violation and exits
 // non-zero.
-//   1. the child references decoded from every packed node equal the 64-byte node's;
+//   1. the child references decoded from every packed node equal the 64-byte node's,
+//      internal children as byte offsets (index x 48), leaf codes unchanged;
 //   2. every stored box [c - h, c + h] contains the exact one, with h at least
 //      (1 + 2^-10) x the exact half extent about the stored centre; a centre that carries
 //      a byte lies within 255 ulp of the midpoint, and h exceeds the needed value by at
@@ -64,8 +65,8 @@ int main(int argc, char** argv) {
   for (size_t i = 0; i < dn.size(); ++i) {
     const trt::DevNode2& a = dn[i];
     const trt::DevNode48& p = pk[i];
-    if (trt::node_child_ref(p.r0[0], p.r0[1], p.r0[2], p.r0[3]) != a.c[0]) return fail("child0 ref", i);
-    if (trt::node_child_ref(p.r1[0], p.r1[1], p.r1[2], p.r1[3]) != a.c[1]) return fail("child1 ref", i);
+    if (trt::node_child_ref(p.r0[0], p.r0[1], p.r0[2], p.r0[3]) != trt::device_child_ref(a.c[0])) return fail("child0 ref", i);
+    if (trt::node_child_ref(p.r1[0], p.r1[1], p.r1[2], p.r1[3]) != trt::device_child_ref(a.c[1])) return fail("child1 ref", i);
     const float lo[2][3] = {{a.r0[0], a.r0[2], a.r2[0]}, {a.r1[0], a.r1[2], a.r2[2]}};
     const float hi[2][3] = {{a.r0[1], a.r0[3], a.r2[1]}, {a.r1[1], a.r1[3], a.r2[3]}};
     const float cs[2][3] = {{p.r0[0], p.r0[2], p.r2[0]}, {p.r1[0], p.r1[2], p.r2[2]}};

@@ -757,7 +757,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
       in.mesh = ti.mesh;
       in.tri_offset = ctx->meshes[ti.mesh].tri_offset;
       in.mat_offset = ti.material_offset;
-      in.blas_root = blas_root[ti.mesh];
+      in.blas_root = blas_root[ti.mesh] * kNodeBytes;  // byte offset, like every node reference
       in.id = i;
       const Mesh& m = ctx->meshes[ti.mesh];
       for (uint32_t k = 0; k < m.n_tris; ++k)
@@ -812,9 +812,9 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
     ctx->sky.sun_power = sun_selection_power(ctx->sky, ctx->scene_radius);
   }
 
-  // the traversal addresses nodes by 32-bit byte offsets
-  if (nodes.size() >= (uint64_t(1) << 32) / sizeof(DevNode48) || tlas_nodes.size() >= (uint64_t(1) << 32) / sizeof(DevNode48))
-    return fail(ctx, TRT_UNSUPPORTED, "BVH too large: more than 89M nodes in one tree array");
+  // child references are byte offsets below 2^31
+  if (nodes.size() >= kMaxDeviceNodes || tlas_nodes.size() >= kMaxDeviceNodes)
+    return fail(ctx, TRT_UNSUPPORTED, "BVH too large: more than 44M nodes in one tree array");
   std::vector<DevNode48> nodes48, tlas48;
   if (!pack_nodes(nodes, nodes48) || !pack_nodes(tlas_nodes, tlas48))
     return fail(ctx, TRT_UNSUPPORTED, "scene bounds too large for the node layout (infinite coordinates)");
@@ -836,6 +836,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
   }
   HIPC(hipStreamSynchronize(s), "upload");
 
+  if (!traversal_lds_check()) return fail(ctx, TRT_DEVICE, "traversal kernels have static LDS: the stack rings need dynamic LDS at 0");
   // persistent grid sizes from occupancy (every resident slot gets a wave)
   ctx->grid_trace = ctx->num_cus * trace_blocks_per_cu(accel_of(ctx));
   ctx->grid_shade = ctx->num_cus * 8;

@@ -451,6 +451,8 @@ bool centre_half(float lo, float hi, int cb, int hb, float& c, float& h) {
 }  // namespace
 
 bool pack_nodes(const std::vector<DevNode2>& in, std::vector<DevNode48>& out) {
+  static_assert(sizeof(DevNode48) == kNodeBytes, "node size");
+  if (in.size() >= kMaxDeviceNodes) return false;
   out.resize(in.size());
   for (size_t i = 0; i < in.size(); ++i) {
     const DevNode2& n = in[i];
@@ -458,7 +460,7 @@ bool pack_nodes(const std::vector<DevNode2>& in, std::vector<DevNode48>& out) {
     const float* xy[2] = {n.r0, n.r1};  // lo.x, hi.x, lo.y, hi.y per child
     float* row[2] = {d.r0, d.r1};
     for (int c = 0; c < 2; ++c) {
-      const uint32_t ref = uint32_t(n.c[c]);
+      const uint32_t ref = uint32_t(device_child_ref(n.c[c]));
       for (int k = 0; k < 2; ++k) {
         const int cb = int((ref >> (16 * k)) & 0xffu), hb = int((ref >> (16 * k + 8)) & 0xffu);
         if (!centre_half(xy[c][2 * k], xy[c][2 * k + 1], cb, hb, row[c][2 * k], row[c][2 * k + 1])) return false;

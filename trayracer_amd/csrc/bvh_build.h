@@ -71,18 +71,19 @@ inline int32_t leaf_code(uint32_t first, uint32_t count) { return ~static_cast<i
 // extent per axis,
 //   row0 = (c0.cx, c0.hx, c0.cy, c0.hy), row1 = (c1.cx, c1.hx, c1.cy, c1.hy),
 //   row2 = (c0.cz, c0.hz, c1.cz, c1.hz),
-// so that a slab's near and far distances are ct - |ht| and ct + |ht| with
-// ct = (c - o) * inv and ht = h * inv: floating-point adds and multiplies only, with no
-// per-axis min / max (which issue at half the rate of FP32 arithmetic on gfx950). The two
-// child references ride in the low bytes of the x and y values (child0 in row0's four
-// floats, child1 in row1's, least significant byte first), so a node is three 16-byte
-// loads. A centre that carries a byte is moved down to the nearest float with that low
-// byte (at most 255 ulp); the half extent then covers the exact box from the stored
-// centre on both sides, is enlarged by 2^-10 of itself (which covers the rounding of
-// ct - |ht| and ct + |ht| beyond the relative error the traversal's far-distance slack
-// covers) and, if it carries a byte, moved up to the nearest float with that low byte.
-// Every stored box therefore contains the exact one and the slab test stays
-// conservative.
+// so that a slab's near and far distances are (x - s h) inv and (x + s h) inv with
+// x = c - o and s = sign(inv): floating-point adds and multiplies only, with no per-axis
+// min / max (which issue at half the rate of FP32 arithmetic on gfx950). The two child
+// references ride in the low bytes of the x and y values (child0 in row0's four floats,
+// child1 in row1's, least significant byte first), so a node is three 16-byte loads. An
+// internal child is referenced by its byte offset in the node array (index x 48, so the
+// traversal forms the address with one add; offsets stay below 2^31), a leaf by its
+// leaf code. A centre that carries a byte is moved down to the nearest float with that
+// low byte (at most 255 ulp); the half extent then covers the exact box from the stored
+// centre on both sides, is enlarged by 2^-10 of itself (which covers the rounding of the
+// slab distances beyond the relative error the traversal's far-distance slack covers)
+// and, if it carries a byte, moved up to the nearest float with that low byte. Every
+// stored box therefore contains the exact one and the slab test stays conservative.
 struct alignas(16) DevNode48 {
   float r0[4], r1[4], r2[4];
 };
@@ -93,11 +94,17 @@ TRT_FI int32_t node_child_ref(float a, float b, float c, float d) {
                               (f2u(d) << 24));
 }
 
+// Device reference of a DevNode2 child: the byte offset of an internal node, a leaf code
+// unchanged.
+constexpr uint32_t kNodeBytes = 48;
+constexpr uint64_t kMaxDeviceNodes = (uint64_t(1) << 31) / kNodeBytes;  // offsets below 2^31
+inline int32_t device_child_ref(int32_t c) { return c >= 0 ? c * static_cast<int32_t>(kNodeBytes) : c; }
+
 // Relative enlargement of every stored half extent.
 constexpr double kHalfExtentMargin = 1.0 / 1024.0;
 
-// Converts nodes to the 48-byte layout. False for a non-finite bound or one that cannot
-// carry its byte.
+// Converts nodes to the 48-byte layout. False for a non-finite bound, one that cannot
+// carry its byte, or kMaxDeviceNodes nodes or more.
 bool pack_nodes(const std::vector<DevNode2>& in, std::vector<DevNode48>& out);
 
 // Converts a build tree into the device layout (depth-first order, node 0 = root).

@@ -131,6 +131,7 @@ constexpr uint32_t kGradMatsPerPass = 512;
 
 // Resident 256-thread blocks per CU of the traversal kernels for this accel's LDS stack.
 int trace_blocks_per_cu(const DevAccel& acc);
+bool traversal_lds_check();  // false if a queue traversal kernel has static LDS
 
 // Kernel launchers (kernels.hip). All run on `stream`. `cap` is the per-segment
 // capacity of every queue; out_cnt / shadow_cnt point at kSegs counter lines.

@@ -199,14 +199,13 @@ __device__ __forceinline__ int32_t row_child_ref(float4 r) {
   return static_cast<int32_t>(lo | hi);
 }
 
-// One node: three 16-byte loads, child references from the x / y values' low bytes.
+// One node at byte offset i (node references are byte offsets): three 16-byte loads,
+// child references from the x / y values' low bytes.
 __device__ __forceinline__ void load_node(const DevNode48* __restrict__ nodes, int32_t i, float4& r0, float4& r1,
                                           float4& r2, int4& c) {
-  // a 32-bit byte offset from the (wave-uniform) array base: 32-bit arithmetic and a load
-  // with a scalar base instead of 64-bit address arithmetic per visit (trt_build keeps
-  // node arrays below 4 GiB)
-  const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) +
-                                                    static_cast<uint32_t>(i) * 48u);
+  // a 32-bit offset from the (wave-uniform) array base: one add per visit and a load with
+  // a scalar base, instead of 64-bit address arithmetic
+  const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) + static_cast<uint32_t>(i));
   r0 = p[0];
   r1 = p[1];
   r2 = p[2];
@@ -302,9 +301,24 @@ constexpr int32_t kTravDone = static_cast<int32_t>(0x80000000u);
 constexpr uint32_t kSearchBreak = 8;
 constexpr uint32_t kSearchBreakSpread = 16;
 
+// End of a queue traversal's node loop, evaluated by all lanes after each iteration as a
+// wave-uniform branch (two compares and scalar work): no lane has an inner node left
+// (walks), or at most kSearchBreakSpread of the lanes that walked in this iteration
+// (walked) hold no parked leaf.
+__device__ __forceinline__ bool walk_done(bool walked, bool walks, int32_t leaf) {
+  const uint64_t searching = __ballot(walked) & __ballot(leaf == 0);
+  // two 32-bit counts: a 64-bit compare of the count would be a vector instruction
+  const uint32_t n = static_cast<uint32_t>(__builtin_popcount(static_cast<uint32_t>(searching))) +
+                     static_cast<uint32_t>(__builtin_popcount(static_cast<uint32_t>(searching >> 32)));
+  return __ballot(walks) == 0 || n <= kSearchBreakSpread;
+}
+
 // Lanes of the wave (among the active ones) that hold no parked leaf.
 __device__ __forceinline__ uint32_t searching_lanes(int32_t leaf) {
-  return static_cast<uint32_t>(__popcll(__ballot(leaf == 0)));
+  const uint64_t m = __ballot(leaf == 0);
+  // two 32-bit counts: a 64-bit compare of the count would be a vector instruction
+  return static_cast<uint32_t>(__builtin_popcount(static_cast<uint32_t>(m))) +
+         static_cast<uint32_t>(__builtin_popcount(static_cast<uint32_t>(m >> 32)));
 }
 
 // Single-level BVH2 traversal in the "speculative while-while" form of Aila & Laine
@@ -420,13 +434,41 @@ constexpr uint32_t kRefill = 40;
 // registers, not by the tree depth.
 constexpr int kRingEntries = 16;
 
+// LDS word at a byte address (ds_read / ds_write with a 32-bit address formed by integer
+// arithmetic, so that a ring slot's address is one and-or of the lane's ring base and
+// the slot offset).
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+__device__ __forceinline__ uint32_t lds_address(uint32_t* p) {
+  return static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_u32*)p));  // address-space cast
+}
+__device__ __forceinline__ uint32_t lds_load(uint32_t a) { return *reinterpret_cast<lds_u32*>(static_cast<uintptr_t>(a)); }
+__device__ __forceinline__ void lds_store(uint32_t a, uint32_t v) {
+  *reinterpret_cast<lds_u32*>(static_cast<uintptr_t>(a)) = v;
+}
+
+// Entry counts are kept in units of kSlotBytes (the ring's slot stride: 64 lanes x 4 bytes),
+// so that entry k's ring slot is at ring | (k * kSlotBytes & kRingMask). The wave's ring is
+// 4 KiB aligned in LDS (the traversal kernels have no static LDS: dynamic LDS starts at 0,
+// checked at start-up by traversal_lds_check), so the slot bits never meet the lane's base.
+constexpr uint32_t kSlotBytes = 64 * 4;
+constexpr uint32_t kRingMask = (kRingEntries - 1) * kSlotBytes;
+
 struct TwoTierStack {
-  uint32_t* ring;        // lane's first ring slot (slots 64 words apart)
+  uint32_t ring;         // LDS byte address of the lane's first ring slot
   uint32_t* spill_wave;  // the wave's spill area; lane l's `cap` entries start at l * cap
-  int cap;
-  int sp;                // entries on the stack
-  int lo;                // entries [0, lo) are in the spill area, [lo, sp) in the ring
+  uint32_t cap;          // capacity (entries)
+  uint32_t sp;           // entries on the stack, x kSlotBytes
+  uint32_t lo;           // entries [0, lo) are in the spill area, [lo, sp) in the ring (x kSlotBytes)
+  uint32_t lim;          // min(lo + kRingEntries, cap) x kSlotBytes: a push at sp == lim either moves
+                         // the ring's oldest entry out or, at the capacity, replaces the top entry
 };
+
+__device__ __forceinline__ void stack_reset(TwoTierStack& s) {
+  s.sp = 0;
+  s.lo = 0;
+  s.lim = min(static_cast<uint32_t>(kRingEntries), s.cap) * kSlotBytes;
+}
 
 // The wave's spill area of the traversal grid (stack_cap entries per thread), taken
 // wave-uniform so that it stays in scalar registers; a lane's own area is formed only
@@ -436,31 +478,41 @@ __device__ __forceinline__ uint32_t* wave_spill(uint32_t* spill, int cap) {
   return spill + static_cast<size_t>(blockIdx.x * kBlock + wave_first) * static_cast<uint32_t>(cap);
 }
 
-__device__ __forceinline__ uint32_t* lane_spill(const TwoTierStack& s) {
-  return s.spill_wave + lane_id() * static_cast<uint32_t>(s.cap);
-}
+__device__ __forceinline__ uint32_t* lane_spill(const TwoTierStack& s) { return s.spill_wave + lane_id() * s.cap; }
 
-template <bool STATS>
-__device__ __forceinline__ void stack_push(TwoTierStack& s, uint32_t v, TraceOut& stats) {
-  if (s.sp - s.lo == kRingEntries) {  // ring full: move its oldest entry out
-    lane_spill(s)[s.lo] = s.ring[(s.lo & (kRingEntries - 1)) * 64];
-    ++s.lo;
-    if (STATS) stats.spills++;
+// Pushes v. At the capacity (a tree deeper than the stack was sized for) v replaces the
+// top entry, which keeps every access in bounds, and on_full() reports the inexact
+// traversal.
+template <bool STATS, class OnFull>
+__device__ __forceinline__ void stack_push(TwoTierStack& s, uint32_t v, TraceOut& stats, OnFull on_full) {
+  if (s.sp == s.lim) {
+    if (s.sp == s.cap * kSlotBytes) {
+      on_full();
+      s.sp -= kSlotBytes;
+    } else {  // ring full: move its oldest entry out
+      lane_spill(s)[s.lo / kSlotBytes] = lds_load(s.ring | (s.lo & kRingMask));
+      s.lo += kSlotBytes;
+      s.lim = min(s.lo + kRingEntries * kSlotBytes, s.cap * kSlotBytes);
+      if (STATS) stats.spills++;
+    }
   }
-  s.ring[(s.sp & (kRingEntries - 1)) * 64] = v;
-  ++s.sp;
+  lds_store(s.ring | (s.sp & kRingMask), v);
+  s.sp += kSlotBytes;
 }
 
 __device__ __forceinline__ int32_t stack_pop(TwoTierStack& s) {
-  if (s.sp == 0) return kTravDone;
-  if (s.sp == s.lo) {  // ring empty: bring back up to a ring of spilled entries
-    const int n = s.lo < kRingEntries ? s.lo : kRingEntries;
+  if (s.sp == s.lo) {
+    if (s.sp == 0) return kTravDone;
+    // ring empty: bring back up to a ring of spilled entries
+    const uint32_t lo = s.lo / kSlotBytes;
+    const uint32_t n = lo < kRingEntries ? lo : kRingEntries;
     const uint32_t* sp = lane_spill(s);
-    for (int k = s.lo - n; k < s.lo; ++k) s.ring[(k & (kRingEntries - 1)) * 64] = sp[k];
-    s.lo -= n;
+    for (uint32_t k = lo - n; k < lo; ++k) lds_store(s.ring | ((k * kSlotBytes) & kRingMask), sp[k]);
+    s.lo -= n * kSlotBytes;
+    s.lim = min(s.lo + kRingEntries * kSlotBytes, s.cap * kSlotBytes);
   }
-  --s.sp;
-  return static_cast<int32_t>(s.ring[(s.sp & (kRingEntries - 1)) * 64]);
+  s.sp -= kSlotBytes;
+  return static_cast<int32_t>(lds_load(s.ring | (s.sp & kRingMask)));
 }
 
 // Leaf phase of the single-level loop, spread over the whole wave: every (ray, triangle)
@@ -611,11 +663,10 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
   r.prim = kNoHit;
   r.inst = kNoInstance;
   TwoTierStack st;
-  st.ring = stk;
+  st.ring = lds_address(stk);
   st.spill_wave = wave_spill(acc.spill, cap);
-  st.cap = cap;
-  st.sp = 0;
-  st.lo = 0;
+  st.cap = static_cast<uint32_t>(cap);
+  stack_reset(st);
   int32_t node = kTravDone, leaf = 0;
   auto pop = [&]() -> int32_t { return stack_pop(st); };
   const LeafTasks tk = leaf_tasks(tsk);
@@ -637,8 +688,7 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
           r.t = tmax;
           r.prim = kNoHit;
           r.inst = kNoInstance;
-          st.sp = 0;
-          st.lo = 0;
+          stack_reset(st);
           node = 0;
           leaf = 0;
         }
@@ -647,15 +697,15 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
       }
     }
     if (!__any(slot != kNoHit)) break;  // drained and every lane idle
-    if (slot != kNoHit) {
-      // inner nodes
-      while (node >= 0) {
+    // inner nodes: a wave-uniform loop in which lanes without an inner node idle
+    for (bool walks = node >= 0;;) {
+      if (walks) {
         float4 r0, r1, r2;
         int4 c;
         load_node(nodes, node, r0, r1, r2, c);
         if (STATS) {
           stats.nodes++;
-          touch(acc.touch_nodes, static_cast<uint32_t>(node));
+          touch(acc.touch_nodes, static_cast<uint32_t>(node) / kNodeBytes);
         }
         float t0, t1;
         const bool h0 = slab_ch(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
@@ -666,11 +716,7 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
             nearc = c.y;
             farc = c.x;
           }
-          if (st.sp < cap) {
-            stack_push<STATS>(st, static_cast<uint32_t>(farc), stats);
-          } else {
-            atomicOr(overflow, 1u);
-          }
+          stack_push<STATS>(st, static_cast<uint32_t>(farc), stats, [&] { atomicOr(overflow, 1u); });
           node = nearc;
         } else if (h0) {
           node = c.x;
@@ -683,8 +729,10 @@ __device__ __forceinline__ void trace_queue(const DevAccel& acc, const SegIn& in
           leaf = node;
           node = pop();
         }
-        if (searching_lanes(leaf) <= kSearchBreakSpread) break;
       }
+      const bool walked = walks;
+      walks = node >= 0;
+      if (walk_done(walked, walks, leaf)) break;
     }
     // leaves (the parked one, and `node` when it is a leaf), spread over the wave
     bool hit = false;
@@ -756,20 +804,19 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
   uint32_t cur_id = kNoInstance;
   bool in_blas = false;
   TwoTierStack st;
-  st.ring = stk;
+  st.ring = lds_address(stk);
   st.spill_wave = wave_spill(acc.spill, cap);
-  st.cap = cap;
-  st.sp = 0;
-  st.lo = 0;
+  st.cap = static_cast<uint32_t>(cap);
+  stack_reset(st);
   int32_t node = kTravDone, leaf = 0;
   bool ok = true;  // false after a stack overflow (reported once per lane)
   auto push = [&](uint32_t v) {
-    if (st.sp < cap) {
-      stack_push<STATS>(st, v, stats);
-    } else if (ok) {
-      ok = false;
-      atomicOr(overflow, 1u);
-    }
+    stack_push<STATS>(st, v, stats, [&] {
+      if (ok) {
+        ok = false;
+        atomicOr(overflow, 1u);
+      }
+    });
   };
   // Pops the next node; leaves the instance at its marker unless a leaf is parked.
   auto pop = [&]() -> int32_t {
@@ -777,7 +824,7 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
       const int32_t e = stack_pop(st);
       if (e != kTravExit) return e;
       if (leaf != 0) {
-        stack_push<STATS>(st, static_cast<uint32_t>(kTravExit), stats);
+        stack_push<STATS>(st, static_cast<uint32_t>(kTravExit), stats, [] {});  // just popped: never full
         return kTravBlocked;
       }
       in_blas = false;
@@ -825,8 +872,7 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
           r.t = tmax;
           r.prim = kNoHit;
           r.inst = kNoInstance;
-          st.sp = 0;
-          st.lo = 0;
+          stack_reset(st);
           node = 0;
           leaf = 0;
         }
@@ -836,14 +882,15 @@ __device__ __forceinline__ void trace_queue2(const DevAccel& acc, const SegIn& i
     }
     if (!__any(slot != kNoHit)) break;
     if (slot != kNoHit) {
-      // inner nodes of either level
+      // inner nodes of either level (a while loop: the uniform form of the single-level
+      // loop costs this kernel register spills)
       while (node >= 0) {
         float4 r0, r1, r2;
         int4 c;
         load_node(in_blas ? blas : tlas, node, r0, r1, r2, c);
         if (STATS) {
           stats.nodes++;
-          if (in_blas) touch(acc.touch_nodes, static_cast<uint32_t>(node));
+          if (in_blas) touch(acc.touch_nodes, static_cast<uint32_t>(node) / kNodeBytes);
         }
         float t0, t1;
         const bool h0 = slab_ch(r0.x, r0.y, r0.z, r0.w, r2.x, r2.y, rb, tmin, r.t, t0);
@@ -1612,6 +1659,20 @@ int trace_blocks_per_cu(const DevAccel& acc) {
   if (eb != hipSuccess) b = 4;
   const int m = a < b ? a : b;
   return m > 0 ? m : 1;
+}
+
+bool traversal_lds_check() {
+  // The queue traversals address their LDS rings as 4 KiB-aligned blocks from LDS
+  // address 0: they must have no static LDS, which would move dynamic LDS up.
+  const void* ks[] = {reinterpret_cast<const void*>(k_extend_sl<false>), reinterpret_cast<const void*>(k_extend_sl<true>),
+                      reinterpret_cast<const void*>(k_shadow_sl<false>), reinterpret_cast<const void*>(k_shadow_sl<true>),
+                      reinterpret_cast<const void*>(k_extend_tl<false>), reinterpret_cast<const void*>(k_extend_tl<true>),
+                      reinterpret_cast<const void*>(k_shadow_tl<false>), reinterpret_cast<const void*>(k_shadow_tl<true>)};
+  for (const void* k : ks) {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, k) != hipSuccess || a.sharedSizeBytes != 0) return false;
+  }
+  return true;
 }
 
 void launch_zero_counters(uint32_t* ctr, hipStream_t stream) {

@@ -31,7 +31,7 @@ struct alignas(16) Instance {
   uint32_t mesh;        // mesh index
   uint32_t tri_offset;  // first triangle of the mesh in the global triangle arrays
   uint32_t mat_offset;  // added to the mesh's per-triangle material ids
-  uint32_t blas_root;   // root node of the mesh's bottom-level BVH (device layout)
+  uint32_t blas_root;   // byte offset of the root of the mesh's bottom-level BVH in the node array
   uint32_t id;          // caller's instance index (hit reports and tie order use it)
   uint32_t pad[3];
 };
