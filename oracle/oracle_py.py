@@ -48,6 +48,8 @@ _PROTO = [
     ("orc_alias_build", None, [_d, ctypes.c_uint32, _f, _u]),
     ("orc_bsdf_eval", None, [_f, _f, _f, ctypes.c_uint32, _f, _f]),
     ("orc_bsdf_sample", None, [_f, _f, _f, ctypes.c_uint32, _f, _f]),
+    ("orc_bsdf_eval_fused", None, [_f, _f, _f, ctypes.c_uint32, _f, _f]),
+    ("orc_bsdf_sample_fused", None, [_f, _f, _f, ctypes.c_uint32, _f, _f]),
     ("orc_intersect", None, [_f, _f, ctypes.c_uint32, _f]),
     ("orc_camera_rays", None, [_f, ctypes.c_uint32, ctypes.c_uint32, _u, _f, ctypes.c_uint32, _f]),
     ("orc_ggx_d", None, [ctypes.c_float, ctypes.c_float, _f, ctypes.c_uint32, _f]),
@@ -331,6 +333,30 @@ def bsdf_sample(mat, wo, u):
     wi = np.empty((n, 3), np.float32)
     pdf = np.empty(n, np.float32)
     load_oracle().orc_bsdf_sample(fp(mat), fp(wo), fp(u), n, fp(wi), fp(pdf))
+    return wi, pdf
+
+
+def bsdf_eval_fused(mat, wo, wi):
+    """(f, pdf) from the integrator's one-pass evaluation (bsdf_f_pdf)."""
+    mat = np.ascontiguousarray(mat, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    n = wo.shape[0]
+    f = np.empty((n, 3), np.float32)
+    pdf = np.empty(n, np.float32)
+    load_oracle().orc_bsdf_eval_fused(fp(mat), fp(wo), fp(wi), n, fp(f), fp(pdf))
+    return f, pdf
+
+
+def bsdf_sample_fused(mat, wo, u):
+    """(wi, pdf) from bsdf_sample_dir and the one-pass evaluation."""
+    mat = np.ascontiguousarray(mat, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 3)
+    n = wo.shape[0]
+    wi = np.empty((n, 3), np.float32)
+    pdf = np.empty(n, np.float32)
+    load_oracle().orc_bsdf_sample_fused(fp(mat), fp(wo), fp(u), n, fp(wi), fp(pdf))
     return wi, pdf
 
 

@@ -250,3 +250,32 @@ def test_anisotropic_ggx_is_normalised(ax, ay):
     # the aspect: D falls off faster along the axis of the smaller alpha
     d = O.ggx_d(ax, ay, np.array([[math.sin(0.1), 0, math.cos(0.1)], [0, math.sin(0.1), math.cos(0.1)]]))
     assert (d[0] > d[1]) == (ax > ay) or ax == ay
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+def test_one_pass_evaluation_equals_separate_functions(name):
+    """The integrator's bsdf_f_pdf and bsdf_sample_dir (used when evaluation and sampling
+    parameters coincide) give bsdf_f, bsdf_pdf and bsdf_sample's results bit for bit,
+    including directions below the horizon and grazing ones."""
+    rng = np.random.default_rng(7)
+    m = MATS[name]
+    n = 20000
+    wo = rng.normal(size=(n, 3)).astype(np.float32)
+    wi = rng.normal(size=(n, 3)).astype(np.float32)
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    wo[:200, 2] = 0.0
+    wi[200:400, 2] = 1e-7
+    wi[400:600] = wo[400:600]  # wi = wo: half vector = wo
+    f, pdf = O.bsdf_eval(m, wo, wi)
+    f1, pdf1 = O.bsdf_eval_fused(m, wo, wi)
+    assert np.array_equal(f.view(np.uint32), f1.view(np.uint32))
+    assert np.array_equal(pdf.view(np.uint32), pdf1.view(np.uint32))
+    assert (pdf > 0).sum() > n // 4
+    u = rng.random((n, 3)).astype(np.float32)
+    s_wi, s_pdf = O.bsdf_sample(m, np.abs(wo), u)
+    d_wi, d_pdf = O.bsdf_sample_fused(m, np.abs(wo), u)
+    ok = s_pdf > 0
+    assert ok.sum() > n // 2
+    assert np.array_equal(s_pdf.view(np.uint32), d_pdf.view(np.uint32))
+    assert np.array_equal(s_wi[ok].view(np.uint32), d_wi[ok].view(np.uint32))

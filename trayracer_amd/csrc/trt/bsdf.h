@@ -223,6 +223,84 @@ TRT_HD inline float bsdf_pdf(const BsdfParams<float>& p, V3 wo, V3 wi) {
   return pdf;
 }
 
+// bsdf_f(p, wo, wi) and bsdf_pdf(p, wo, wi) of one parameter set in one pass, for the
+// integrator where the evaluation and sampling parameters coincide: the half vector, the
+// GGX alphas, D, Lambda(wo) and the clearcoat D are formed once instead of twice. Every
+// value is formed by the same operations as in the two separate functions, so both
+// results equal theirs bit for bit.
+TRT_HD inline T3<float> bsdf_f_pdf(const BsdfParams<float>& p, V3 wo, V3 wi, float& pdf) {
+  pdf = 0.f;
+  const float zero = 0.f;
+  T3<float> out = t3(zero, zero, zero);
+  if (wo.z <= 0.f || wi.z <= 0.f) return out;
+  if (p.model == kModelLambert) {
+    pdf = wi.z * kInvPi;
+    return p.base * kInvPi;
+  }
+  const V3 h = normalize(wo + wi);
+  const float cl = wi.z, cv = wo.z, cd = dot(wi, h);
+  const float FL = schlick_weight(cl), FV = schlick_weight(cv), FH = schlick_weight(cd);
+  const float one_m_metal = 1.f - p.metallic;
+
+  const float lum = lum3(p.base);
+  T3<float> tint = t3(1.f, 1.f, 1.f);
+  if (lum > 0.f) tint = t3(p.base.x / lum, p.base.y / lum, p.base.z / lum);
+
+  const float rcd2 = p.roughness * (cd * cd);
+  const float fd90 = 0.5f + 2.f * rcd2;
+  const float fd = (1.f + (fd90 - 1.f) * FL) * (1.f + (fd90 - 1.f) * FV);
+  float dw = fd;
+  if (p.subsurface > 0.f) {
+    const float fss = (1.f + (rcd2 - 1.f) * FL) * (1.f + (rcd2 - 1.f) * FV);
+    const float ss = 1.25f * (fss * (1.f / (cl + cv) - 0.5f) + 0.5f);
+    dw = (1.f - p.subsurface) * fd + p.subsurface * ss;
+  }
+  const float kd = dw * kInvPi * one_m_metal;
+  out = p.base * kd;
+
+  if (p.sheen > 0.f) {
+    const float ks = one_m_metal * (FH * p.sheen);
+    const float st = p.sheen_tint;
+    out = out + t3(((1.f - st) + st * tint.x) * ks, ((1.f - st) + st * tint.y) * ks, ((1.f - st) + st * tint.z) * ks);
+  }
+
+  float ax, ay;
+  ggx_alpha(p.roughness, p.anisotropic, ax, ay);
+  const float D = ggx_d(h, ax, ay);
+  const float l1 = 1.f + ggx_lambda(wo, ax, ay);  // 1 + Lambda(wo): G's and the pdf's
+  const float G = 1.f / (l1 + ggx_lambda(wi, ax, ay));
+  const float spec_common = D * G / (4.f * cl * cv);
+  const float sp = p.specular * 0.08f, stt = p.specular_tint;
+  const T3<float> c0 = t3(mixT(0.f, p.base.x, p.metallic) + (1.f - p.metallic) * (sp * ((1.f - stt) + stt * tint.x)),
+                          mixT(0.f, p.base.y, p.metallic) + (1.f - p.metallic) * (sp * ((1.f - stt) + stt * tint.y)),
+                          mixT(0.f, p.base.z, p.metallic) + (1.f - p.metallic) * (sp * ((1.f - stt) + stt * tint.z)));
+  const T3<float> F = t3(c0.x + (1.f - c0.x) * FH, c0.y + (1.f - c0.y) * FH, c0.z + (1.f - c0.z) * FH);
+  out = out + F * spec_common;
+
+  const float a = lerp(0.1f, 0.001f, p.clearcoat_gloss);
+  float Dr = 0.f;
+  if (p.clearcoat > 0.f) {
+    Dr = gtr1_d(h, a);
+    const float Fr = lerp(0.04f, 1.f, FH);
+    const float Gr = smith_g1_ggx_iso(cl, 0.25f) * smith_g1_ggx_iso(cv, 0.25f);
+    const float kc = 0.25f * p.clearcoat * Dr * Fr * Gr / (4.f * cl * cv);
+    out = out + t3(zero + kc, zero + kc, zero + kc);
+  }
+
+  // bsdf_pdf's terms
+  float pd, ps, pc;
+  lobe_probs(p, wo.z, pd, ps, pc);
+  const float oh = dot(wo, h);
+  float q = pd * (wi.z * kInvPi);
+  if (oh > 0.f) {
+    const float g1 = 1.f / l1;
+    q += ps * (g1 * D / (4.f * wo.z));
+    if (pc > 0.f) q += pc * ((p.clearcoat > 0.f ? Dr : gtr1_d(h, a)) * h.z / (4.f * oh));
+  }
+  pdf = q;
+  return out;
+}
+
 // Heitz 2018 VNDF sample of the anisotropic GGX distribution (wo.z > 0)
 TRT_FI V3 sample_ggx_vndf(V3 wo, float ax, float ay, float u0, float u1) {
   const V3 vh = normalize(v3(ax * wo.x, ay * wo.y, wo.z));
@@ -258,21 +336,15 @@ struct BsdfSample {
   float pdf;  // 0 = no sample
 };
 
-// Draws wi given three uniforms: ul selects the lobe, (u0,u1) the direction.
-TRT_HD inline BsdfSample bsdf_sample(const BsdfParams<float>& p, V3 wo, float ul, float u0, float u1) {
-  BsdfSample s;
-  s.pdf = 0.f;
-  s.wi = v3(0.f, 0.f, 1.f);
-  if (wo.z <= 0.f) return s;
+// Direction of bsdf_sample (false: no sample).
+TRT_HD inline bool bsdf_sample_dir(const BsdfParams<float>& p, V3 wo, float ul, float u0, float u1, V3& wi) {
+  if (wo.z <= 0.f) return false;
   if (p.model == kModelLambert) {
-    s.wi = sample_cosine_hemisphere(u0, u1);
-    s.pdf = s.wi.z * kInvPi;
-    if (!(s.wi.z > 0.f)) s.pdf = 0.f;
-    return s;
+    wi = sample_cosine_hemisphere(u0, u1);
+    return wi.z > 0.f;
   }
   float pd, ps, pc;
   lobe_probs(p, wo.z, pd, ps, pc);
-  V3 wi;
   if (ul < pd) {
     wi = sample_cosine_hemisphere(u0, u1);
   } else {
@@ -286,7 +358,23 @@ TRT_HD inline BsdfSample bsdf_sample(const BsdfParams<float>& p, V3 wo, float ul
     }
     wi = reflect(wo, m);
   }
-  if (!(wi.z > 0.f)) return s;
+  return wi.z > 0.f;
+}
+
+// Draws wi given three uniforms: ul selects the lobe, (u0,u1) the direction.
+TRT_HD inline BsdfSample bsdf_sample(const BsdfParams<float>& p, V3 wo, float ul, float u0, float u1) {
+  BsdfSample s;
+  s.pdf = 0.f;
+  s.wi = v3(0.f, 0.f, 1.f);
+  if (wo.z <= 0.f) return s;
+  if (p.model == kModelLambert) {
+    s.wi = sample_cosine_hemisphere(u0, u1);
+    s.pdf = s.wi.z * kInvPi;
+    if (!(s.wi.z > 0.f)) s.pdf = 0.f;
+    return s;
+  }
+  V3 wi;
+  if (!bsdf_sample_dir(p, wo, ul, u0, u1, wi)) return s;
   s.wi = wi;
   s.pdf = bsdf_pdf(p, wo, wi);
   return s;

@@ -261,8 +261,9 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
     if (ok) {
       const V3 wi = to_local(fr, wl);
       if (wi.z > 0.f) {
-        const T3<float> f = bsdf_f(bp, wo, wi);
-        const float pb = bsdf_pdf(bs, wo, wi);
+        float pb;
+        const T3<float> f = rp.separate_sampling ? bsdf_f(bp, wo, wi) : bsdf_f_pdf(bp, wo, wi, pb);
+        if (rp.separate_sampling) pb = bsdf_pdf(bs, wo, wi);
         // balance heuristic against BSDF sampling, whose sample would be traced by the
         // extension ray; the last vertex traces none, so there the light sample stands
         // alone (a partial weight would drop the BSDF share of the last bounce's light)
@@ -289,11 +290,24 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
 
   // extension
   if (depth + 1 < rp.max_depth) {
-    const BsdfSample smp = bsdf_sample(bs, wo, u[4], u[5], u[6]);
+    // with one parameter set, the sample's pdf and f come from one fused evaluation
+    BsdfSample smp;
+    T3<float> f1 = t3(0.f, 0.f, 0.f);
+    if (rp.separate_sampling) {
+      smp = bsdf_sample(bs, wo, u[4], u[5], u[6]);
+    } else {
+      smp.pdf = 0.f;
+      smp.wi = v3(0.f, 0.f, 1.f);
+      V3 wi;
+      if (bsdf_sample_dir(bs, wo, u[4], u[5], u[6], wi)) {
+        smp.wi = wi;
+        f1 = bsdf_f_pdf(bp, wo, wi, smp.pdf);
+      }
+    }
     if (smp.pdf > 0.f) {
       const V3 wiw = normalize(to_world(fr, smp.wi));
       if (dot(wiw, sp.ng) > 0.f) {
-        const T3<float> f = bsdf_f(bp, wo, smp.wi);
+        const T3<float> f = rp.separate_sampling ? bsdf_f(bp, wo, smp.wi) : f1;
         const float k = smp.wi.z / smp.pdf;
         V3 Tn = v3(ps.T.x * f.x * k, ps.T.y * f.y * k, ps.T.z * f.z * k);
         bool alive = !is_black(Tn);
