@@ -900,14 +900,15 @@ void orc_bsdf_sample(const float* mat, const float* wo, const float* u, uint32_t
     pdf[i] = s.pdf;
   }
 }
-// The integrator's one-pass forms: bsdf_f_pdf for (f, pdf), and bsdf_sample_dir followed
-// by bsdf_f_pdf for a sample (wi, pdf; pdf 0 and wi = +z when there is none).
+// The integrator's one-pass forms (per-vertex terms from bsdf_pre): bsdf_f_pdf for
+// (f, pdf), and bsdf_sample_dir followed by bsdf_f_pdf for a sample (wi, pdf; pdf 0 and
+// wi = +z when there is none).
 void orc_bsdf_eval_fused(const float* mat, const float* wo, const float* wi, uint32_t n, float* f, float* pdf) {
   const BsdfParams<float> p = mat_params(mat);
   for (uint32_t i = 0; i < n; ++i) {
     const V3 o = v3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
     const V3 w = v3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
-    const T3<float> v = bsdf_f_pdf(p, o, w, pdf[i]);
+    const T3<float> v = bsdf_f_pdf(p, bsdf_pre(p, o), o, w, pdf[i]);
     f[3 * i] = v.x;
     f[3 * i + 1] = v.y;
     f[3 * i + 2] = v.z;
@@ -920,9 +921,10 @@ void orc_bsdf_sample_fused(const float* mat, const float* wo, const float* u, ui
     V3 w = v3(0.f, 0.f, 1.f);
     float q = 0.f;
     V3 d;
-    if (bsdf_sample_dir(p, o, u[3 * i], u[3 * i + 1], u[3 * i + 2], d)) {
+    const BsdfPre pre = bsdf_pre(p, o);
+    if (bsdf_sample_dir(p, pre, o, u[3 * i], u[3 * i + 1], u[3 * i + 2], d)) {
       w = d;
-      bsdf_f_pdf(p, o, d, q);
+      bsdf_f_pdf(p, pre, o, d, q);
     }
     wi[3 * i] = w.x;
     wi[3 * i + 1] = w.y;

@@ -223,12 +223,34 @@ TRT_HD inline float bsdf_pdf(const BsdfParams<float>& p, V3 wo, V3 wi) {
   return pdf;
 }
 
+// Per-vertex constants of one parameter set and outgoing direction wo (wo.z > 0) that
+// bsdf_f, bsdf_pdf and bsdf_sample would each form again: the tint, the GGX alphas,
+// 1 + Lambda(wo) and the lobe selection probabilities (Disney model; unused for Lambert).
+struct BsdfPre {
+  T3<float> tint;
+  float ax, ay, l1;
+  float pd, ps, pc;
+};
+
+TRT_HD inline BsdfPre bsdf_pre(const BsdfParams<float>& p, V3 wo) {
+  BsdfPre q;
+  q.tint = t3(1.f, 1.f, 1.f);
+  q.ax = q.ay = q.l1 = q.pd = q.ps = q.pc = 0.f;
+  if (p.model == kModelLambert || !(wo.z > 0.f)) return q;
+  const float lum = lum3(p.base);
+  if (lum > 0.f) q.tint = t3(p.base.x / lum, p.base.y / lum, p.base.z / lum);
+  ggx_alpha(p.roughness, p.anisotropic, q.ax, q.ay);
+  q.l1 = 1.f + ggx_lambda(wo, q.ax, q.ay);
+  lobe_probs(p, wo.z, q.pd, q.ps, q.pc);
+  return q;
+}
+
 // bsdf_f(p, wo, wi) and bsdf_pdf(p, wo, wi) of one parameter set in one pass, for the
-// integrator where the evaluation and sampling parameters coincide: the half vector, the
-// GGX alphas, D, Lambda(wo) and the clearcoat D are formed once instead of twice. Every
-// value is formed by the same operations as in the two separate functions, so both
-// results equal theirs bit for bit.
-TRT_HD inline T3<float> bsdf_f_pdf(const BsdfParams<float>& p, V3 wo, V3 wi, float& pdf) {
+// integrator where the evaluation and sampling parameters coincide: the half vector, D
+// and the clearcoat D are formed once instead of twice, and the per-vertex terms come
+// from q = bsdf_pre(p, wo). Every value is formed by the same operations as in the two
+// separate functions, so both results equal theirs bit for bit.
+TRT_HD inline T3<float> bsdf_f_pdf(const BsdfParams<float>& p, const BsdfPre& q, V3 wo, V3 wi, float& pdf) {
   pdf = 0.f;
   const float zero = 0.f;
   T3<float> out = t3(zero, zero, zero);
@@ -241,10 +263,7 @@ TRT_HD inline T3<float> bsdf_f_pdf(const BsdfParams<float>& p, V3 wo, V3 wi, flo
   const float cl = wi.z, cv = wo.z, cd = dot(wi, h);
   const float FL = schlick_weight(cl), FV = schlick_weight(cv), FH = schlick_weight(cd);
   const float one_m_metal = 1.f - p.metallic;
-
-  const float lum = lum3(p.base);
-  T3<float> tint = t3(1.f, 1.f, 1.f);
-  if (lum > 0.f) tint = t3(p.base.x / lum, p.base.y / lum, p.base.z / lum);
+  const T3<float> tint = q.tint;
 
   const float rcd2 = p.roughness * (cd * cd);
   const float fd90 = 0.5f + 2.f * rcd2;
@@ -264,11 +283,9 @@ TRT_HD inline T3<float> bsdf_f_pdf(const BsdfParams<float>& p, V3 wo, V3 wi, flo
     out = out + t3(((1.f - st) + st * tint.x) * ks, ((1.f - st) + st * tint.y) * ks, ((1.f - st) + st * tint.z) * ks);
   }
 
-  float ax, ay;
-  ggx_alpha(p.roughness, p.anisotropic, ax, ay);
+  const float ax = q.ax, ay = q.ay;
   const float D = ggx_d(h, ax, ay);
-  const float l1 = 1.f + ggx_lambda(wo, ax, ay);  // 1 + Lambda(wo): G's and the pdf's
-  const float G = 1.f / (l1 + ggx_lambda(wi, ax, ay));
+  const float G = 1.f / (q.l1 + ggx_lambda(wi, ax, ay));
   const float spec_common = D * G / (4.f * cl * cv);
   const float sp = p.specular * 0.08f, stt = p.specular_tint;
   const T3<float> c0 = t3(mixT(0.f, p.base.x, p.metallic) + (1.f - p.metallic) * (sp * ((1.f - stt) + stt * tint.x)),
@@ -288,16 +305,14 @@ TRT_HD inline T3<float> bsdf_f_pdf(const BsdfParams<float>& p, V3 wo, V3 wi, flo
   }
 
   // bsdf_pdf's terms
-  float pd, ps, pc;
-  lobe_probs(p, wo.z, pd, ps, pc);
   const float oh = dot(wo, h);
-  float q = pd * (wi.z * kInvPi);
+  float pr = q.pd * (wi.z * kInvPi);
   if (oh > 0.f) {
-    const float g1 = 1.f / l1;
-    q += ps * (g1 * D / (4.f * wo.z));
-    if (pc > 0.f) q += pc * ((p.clearcoat > 0.f ? Dr : gtr1_d(h, a)) * h.z / (4.f * oh));
+    const float g1 = 1.f / q.l1;
+    pr += q.ps * (g1 * D / (4.f * wo.z));
+    if (q.pc > 0.f) pr += q.pc * ((p.clearcoat > 0.f ? Dr : gtr1_d(h, a)) * h.z / (4.f * oh));
   }
-  pdf = q;
+  pdf = pr;
   return out;
 }
 
@@ -336,23 +351,20 @@ struct BsdfSample {
   float pdf;  // 0 = no sample
 };
 
-// Direction of bsdf_sample (false: no sample).
-TRT_HD inline bool bsdf_sample_dir(const BsdfParams<float>& p, V3 wo, float ul, float u0, float u1, V3& wi) {
+// Direction of bsdf_sample (false: no sample), with q = bsdf_pre(p, wo).
+TRT_HD inline bool bsdf_sample_dir(const BsdfParams<float>& p, const BsdfPre& q, V3 wo, float ul, float u0, float u1,
+                                   V3& wi) {
   if (wo.z <= 0.f) return false;
   if (p.model == kModelLambert) {
     wi = sample_cosine_hemisphere(u0, u1);
     return wi.z > 0.f;
   }
-  float pd, ps, pc;
-  lobe_probs(p, wo.z, pd, ps, pc);
-  if (ul < pd) {
+  if (ul < q.pd) {
     wi = sample_cosine_hemisphere(u0, u1);
   } else {
     V3 m;
-    if (ul < pd + ps || pc <= 0.f) {
-      float ax, ay;
-      ggx_alpha(p.roughness, p.anisotropic, ax, ay);
-      m = sample_ggx_vndf(wo, ax, ay, u0, u1);
+    if (ul < q.pd + q.ps || q.pc <= 0.f) {
+      m = sample_ggx_vndf(wo, q.ax, q.ay, u0, u1);
     } else {
       m = sample_gtr1(lerp(0.1f, 0.001f, p.clearcoat_gloss), u0, u1);
     }
@@ -374,7 +386,7 @@ TRT_HD inline BsdfSample bsdf_sample(const BsdfParams<float>& p, V3 wo, float ul
     return s;
   }
   V3 wi;
-  if (!bsdf_sample_dir(p, wo, ul, u0, u1, wi)) return s;
+  if (!bsdf_sample_dir(p, bsdf_pre(p, wo), wo, ul, u0, u1, wi)) return s;
   s.wi = wi;
   s.pdf = bsdf_pdf(p, wo, wi);
   return s;

@@ -204,6 +204,9 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
   if (!(wo.z > 0.f)) return;
   const BsdfParams<float> bp = params_of(m);
   const BsdfParams<float> bs = params_of(ms);
+  // per-vertex BSDF terms of the one-pass evaluations (one parameter set)
+  BsdfPre pre{};
+  if (!rp.separate_sampling) pre = bsdf_pre(bp, wo);
   BsdfParams<Dual<kNumParams>> bd;
   if (ADJ) {
     adj->mat = sp.mat;
@@ -262,7 +265,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
       const V3 wi = to_local(fr, wl);
       if (wi.z > 0.f) {
         float pb;
-        const T3<float> f = rp.separate_sampling ? bsdf_f(bp, wo, wi) : bsdf_f_pdf(bp, wo, wi, pb);
+        const T3<float> f = rp.separate_sampling ? bsdf_f(bp, wo, wi) : bsdf_f_pdf(bp, pre, wo, wi, pb);
         if (rp.separate_sampling) pb = bsdf_pdf(bs, wo, wi);
         // balance heuristic against BSDF sampling, whose sample would be traced by the
         // extension ray; the last vertex traces none, so there the light sample stands
@@ -299,9 +302,9 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
       smp.pdf = 0.f;
       smp.wi = v3(0.f, 0.f, 1.f);
       V3 wi;
-      if (bsdf_sample_dir(bs, wo, u[4], u[5], u[6], wi)) {
+      if (bsdf_sample_dir(bs, pre, wo, u[4], u[5], u[6], wi)) {
         smp.wi = wi;
-        f1 = bsdf_f_pdf(bp, wo, wi, smp.pdf);
+        f1 = bsdf_f_pdf(bp, pre, wo, wi, smp.pdf);
       }
     }
     if (smp.pdf > 0.f) {
