@@ -144,6 +144,7 @@ struct OScene {
   OBvh tlas;                   // over instances (two-level)
   std::vector<double> inst_box;
   LightTable lt;
+  SkyParams sky{};  // the sky the view points to
   SceneView view;
   std::vector<Material> samp;  // sampling materials (differentiable renders)
   bool separate_sampling = false;  // set by orc_set_material_params, as trt_set_material_params
@@ -442,6 +443,7 @@ void* orc_scene_create(const float* pos, const float* nrm, const uint32_t* mat_i
   v.total_power = s->lt.total_power;
   v.n_tris = n_tris;
   v.n_materials = n_mats;
+  v.sky = &s->sky;
   s->samp = s->mats;
   return s;
 }
@@ -499,15 +501,15 @@ void orc_set_material_params(void* p, const float* params, uint32_t n, uint32_t 
 void orc_set_sky(void* p, uint32_t enabled, uint32_t draw_sun, const float* dir, float angular_diameter,
                  float intensity, float turbidity, uint32_t sun_nee) {
   OScene* s = static_cast<OScene*>(p);
-  s->view.sky = make_sky(enabled != 0, draw_sun != 0, dir, angular_diameter, intensity, turbidity, sun_nee != 0);
-  s->view.sky.sun_power = sun_selection_power(s->view.sky, s->radius);
+  s->sky = make_sky(enabled != 0, draw_sun != 0, dir, angular_diameter, intensity, turbidity, sun_nee != 0);
+  s->sky.sun_power = sun_selection_power(s->sky, s->radius);
 }
 
 // Sky radiance along n unit directions (3 floats each) -> out (3 floats each).
 void orc_sky_radiance(void* p, const float* dirs, uint32_t n, uint32_t sun, float* out) {
   OScene* s = static_cast<OScene*>(p);
   for (uint32_t i = 0; i < n; ++i) {
-    const V3 r = sky_radiance(s->view.sky, v3(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]), sun != 0);
+    const V3 r = sky_radiance(s->sky, v3(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]), sun != 0);
     out[i * 3] = r.x;
     out[i * 3 + 1] = r.y;
     out[i * 3 + 2] = r.z;
@@ -532,7 +534,7 @@ void orc_tone_curve(const float* x, uint32_t n, uint32_t method, float* out) {
 void orc_scene_destroy(void* p) { delete static_cast<OScene*>(p); }
 
 uint32_t orc_light_count(void* p) { return uint32_t(static_cast<OScene*>(p)->lt.lights.size()); }
-float orc_sun_power(void* p) { return static_cast<OScene*>(p)->view.sky.sun_nee ? static_cast<OScene*>(p)->view.sky.sun_power : 0.f; }
+float orc_sun_power(void* p) { return static_cast<OScene*>(p)->sky.sun_nee ? static_cast<OScene*>(p)->sky.sun_power : 0.f; }
 float orc_total_power(void* p) { return static_cast<OScene*>(p)->lt.total_power; }
 // lights: 20 floats per light (LightTri layout); prob/alias per light
 void orc_light_table(void* p, float* lights, float* prob, uint32_t* alias) {

@@ -137,6 +137,7 @@ struct trt_context {
   // differentiable rendering
   std::vector<Material> sampling_materials;  // empty = same as materials
   DevBuf d_mat_sampling;
+  DevBuf d_sky;  // `sky` as the kernels read it (SceneView::sky)
   bool has_frame = false;      // the last frame's per-path radiance matches the current scene
   uint64_t frame_id = 0;       // frames rendered by this context
   trt_render_opts last_opts{};
@@ -157,6 +158,13 @@ namespace {
 // Scene, camera or material changes make the last frame's per-path radiance stale for
 // the adjoint, which replays that frame against the current state.
 void invalidate_frame(trt_context* c) { c->has_frame = false; }
+
+// Copies ctx->sky to the device copy the kernels read; called whenever it changes (no
+// frame is in flight then: every render call ends synchronised).
+hipError_t upload_sky(trt_context* c) {
+  hipError_t e = hipMemcpyAsync(c->d_sky.p, &c->sky, sizeof(SkyParams), hipMemcpyHostToDevice, c->stream);
+  return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
 
 // Restarts progressive accumulation (Renderer::Reset, Tracer/Renderer/Renderer.cpp:186-190),
 // and with it the denoiser's record of the sample count it last ran at (Denoiser::Reset).
@@ -342,7 +350,7 @@ SceneView view_of(const trt_context* ctx) {
   s.total_power = ctx->total_power;
   s.n_tris = uint32_t(ctx->mat_ids.size());
   s.n_materials = uint32_t(ctx->materials.size());
-  s.sky = ctx->sky;
+  s.sky = ctx->d_sky.as<SkyParams>();
   return s;
 }
 
@@ -537,6 +545,8 @@ static trt_status init_context(trt_context* ctx) {
   for (auto& e : ctx->ev2) HIPC(hipEventCreate(&e), "event");
   HIPC(ctx->counters.ensure(kNumCounters * 4), "alloc counters");
   HIPC(hipMemset(ctx->counters.p, 0, kNumCounters * 4), "clear counters");
+  HIPC(ctx->d_sky.ensure(sizeof(SkyParams)), "alloc sky");
+  HIPC(upload_sky(ctx), "upload sky");
   return TRT_OK;
 }
 
@@ -582,6 +592,7 @@ trt_status trt_clear_scene(trt_context* ctx) {
   ctx->meshes.clear();
   ctx->instances_in.clear();
   ctx->sky = SkyParams{};
+  HIPC(upload_sky(ctx), "upload sky");
   ctx->built = false;
   ctx->dirty = true;
   invalidate_frame(ctx);
@@ -596,6 +607,7 @@ trt_status trt_set_sky(trt_context* ctx, const trt_sky* sky) {
   ctx->sky = make_sky(sky->enabled != 0, sky->draw_sun != 0, sky->sun_dir, sky->angular_diameter, sky->intensity,
                       sky->turbidity, sky->sun_nee != 0);
   ctx->sky.sun_power = sun_selection_power(ctx->sky, ctx->scene_radius);
+  HIPC(upload_sky(ctx), "upload sky");
   reset_accumulation(ctx);
   invalidate_frame(ctx);
   return TRT_OK;
@@ -810,6 +822,7 @@ trt_status trt_build(trt_context* ctx, const trt_build_opts* opts) {
                                      uint32_t(inst_orig.size()), mesh_off.data(), mesh_cnt.data(),
                                      uint32_t(ctx->meshes.size()));
     ctx->sky.sun_power = sun_selection_power(ctx->sky, ctx->scene_radius);
+    HIPC(upload_sky(ctx), "upload sky");
   }
 
   // child references are byte offsets below 2^31

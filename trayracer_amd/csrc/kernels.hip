@@ -1246,18 +1246,23 @@ struct ShadeList {
 // time: in the 1M soup 18% of the rays miss (35% at the second bounce), lanes that the
 // per-slot loop left idle through the whole shading step. Held to 168 VGPRs, 3 waves per
 // SIMD (the allocator otherwise takes 172, 2 waves).
-template <bool DEBUG>
+// SEP: the sampling materials are a set of their own (1) or the evaluation materials (0),
+// fixed per launch from RenderParams::separate_sampling.
+template <bool DEBUG, int SEP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(SceneView sv, const Material* sampling_mats, RenderParams rp,
                                                    RayQueue q, HitBuf h, RayQueue qn, ShadowQueue sq,
                                                    FrameBuffers fb, SegIn in, uint32_t* next_cnt,
                                                    uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth,
-                                                   uint32_t two_level) {
+                                                   uint32_t two_level, const SkyParams* __restrict__ sky) {
+  // sv.sky again, as a restrict kernel argument: nothing the kernel writes can alias it, so
+  // its fields are read with scalar loads where they are used
+  sv.sky = sky;
   __shared__ ShadeList lists[kWavesPerBlock];
   ShadeList& L = lists[threadIdx.x >> 6];
   const uint32_t lane = lane_id();
   const uint32_t npix = rp.width * rp.height;
   const uint32_t n_seg = seg_counts(in), home = wave_home();
-  const bool keep_misses = DEBUG || sv.sky.enabled != 0u;
+  const bool keep_misses = DEBUG || sv.sky->enabled != 0u;
   uint64_t done = seg_empty(n_seg);
   uint32_t pend = 0;  // listed slots (wave-uniform)
   bool more = true;
@@ -1318,7 +1323,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       hit.prim = __float_as_uint(hh.w);
       hit.inst = hinst;
       if (DEBUG) shade_debug(sv, rp, ps, hit, depth, out);
-      else shade(sv, sampling_mats, rp, ps, hit, depth, out);
+      else shade<SEP>(sv, sampling_mats, rp, ps, hit, depth, out);
       if (!is_black(out.L)) {  // rare (emitter hits): this lane is the path's only writer here
         float4 Lr = fb.radiance[path];
         Lr.x += out.L.x;
@@ -1434,7 +1439,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneView sv, const Materi
                                                        RayQueue q, HitBuf h, RayQueue qn, ShadowQueue sq,
                                                        AdjBuffers ab, SegIn in, uint32_t* next_cnt,
                                                        uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth,
-                                                       uint32_t two_level) {
+                                                       uint32_t two_level, const SkyParams* __restrict__ sky) {
+  sv.sky = sky;  // scalar loads of the sky's fields, as in k_shade
   const uint32_t lane = lane_id();
   const uint32_t npix = rp.width * rp.height;
   const float inv_spp = 1.f / static_cast<float>(rp.spp);
@@ -1720,11 +1726,14 @@ void launch_shade(const SceneView& sv, const Material* sampling_mats, const Rend
                   HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, SegIn in, uint32_t* next_cnt,
                   uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream) {
   if (rp.mode != kModePathTracing)
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, fb, in,
-                       next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u);
+    hipLaunchKernelGGL((k_shade<true, -1>), dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq,
+                       fb, in, next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
+  else if (rp.separate_sampling)
+    hipLaunchKernelGGL((k_shade<false, 1>), dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq,
+                       fb, in, next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
   else
-    hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, fb,
-                       in, next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u);
+    hipLaunchKernelGGL((k_shade<false, 0>), dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq,
+                       fb, in, next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
 }
 
 void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,
@@ -1765,7 +1774,7 @@ void launch_shade_adj(const SceneView& sv, const Material* sampling_mats, const 
                       HitBuf h, RayQueue qn, ShadowQueue sq, const AdjBuffers& ab, SegIn in, uint32_t* next_cnt,
                       uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream) {
   hipLaunchKernelGGL(k_shade_adj, dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq, ab, in,
-                     next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u);
+                     next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
 }
 
 void launch_shadow_adj(const DevAccel& acc, ShadowQueue sq, const AdjBuffers& ab, SegIn in, uint32_t* ctr,

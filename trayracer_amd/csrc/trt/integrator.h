@@ -104,7 +104,7 @@ TRT_FI float light_pdf_hit(V3 le, float area, float cos_l, float t, float total)
 }
 
 // Selection weight of the sun disc in next-event estimation (0 when it is not a light).
-TRT_FI float sun_weight(const SceneView& s) { return s.sky.sun_nee ? s.sky.sun_power : 0.f; }
+TRT_FI float sun_weight(const SceneView& s) { return s.sky->sun_nee ? s.sky->sun_power : 0.f; }
 
 // Adjoint extras of one vertex (path-replay backpropagation, Vicini et al. 2021):
 // derivatives of the vertex's terms with respect to its material's differentiable
@@ -118,10 +118,14 @@ struct AdjOut {
   float r[kNumParams][3];       // (d f_c / d theta_k) / f_c at the sampled extension
 };
 
-template <bool ADJ>
+// SEP: whether the sampling materials are a set of their own, -1 = as rp.separate_sampling
+// says (the device kernels fix it at compile time, so that with one set the material is
+// read once and the separate-set code is left out).
+template <bool ADJ, int SEP = -1>
 TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats, const RenderParams& rp,
                               const PathState& ps, const Hit& h, uint32_t depth, ShadeOut& out, V3 delta,
                               AdjOut* adj) {
+  const bool sep = SEP < 0 ? rp.separate_sampling != 0u : SEP == 1;
   out.L = v3(0.f);
   out.has_shadow = 0;
   out.has_ext = 0;
@@ -130,16 +134,16 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
   out.vdepth = 0.f;
   if (ADJ) adj->mat = kNoHit;
   if (h.prim == kNoHit) {  // the ray left the scene: sky radiance (PathTracing.h:40-55)
-    if (s.sky.enabled) {
-      const bool sun = s.sky.draw_sun != 0u && (depth == 0 || (ps.flags & kFlagSpecular));
-      V3 sky = sky_radiance(s.sky, ps.d, sun);
+    if (s.sky->enabled) {
+      const bool sun = s.sky->draw_sun != 0u && (depth == 0 || (ps.flags & kFlagSpecular));
+      V3 sky = sky_radiance(*s.sky, ps.d, sun);
       // a BSDF-sampled ray that reaches the sun disc when the disc is also a light:
       // its radiance with the MIS weight against the sun's next-event estimate
-      if (!sun && s.sky.sun_nee && rp.strategy != 1u) {
-        const V3 ls = sun_radiance(s.sky, ps.d);
+      if (!sun && s.sky->sun_nee && rp.strategy != 1u) {
+        const V3 ls = sun_radiance(*s.sky, ps.d);
         if (!is_black(ls)) {
           const float sw = sun_weight(s);
-          const float pl = (sw / (s.total_power + sw)) * s.sky.sun_pdf;
+          const float pl = (sw / (s.total_power + sw)) * s.sky->sun_pdf;
           const float w = rp.strategy == 2u ? 1.f : ps.pdf / (ps.pdf + pl);
           sky = sky + ls * w;
         }
@@ -177,7 +181,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
 
   const SurfacePoint sp = surface_at(s, h, ps.o, ps.d);
   const Material& m = s.materials[sp.mat];
-  const Material& ms = sampling_mats[sp.mat];
+  const Material& ms = sep ? sampling_mats[sp.mat] : m;  // the same values when not separate
   const V3 le = v3(m.emission[0], m.emission[1], m.emission[2]);
   if (depth == 0) {
     out.albedo = is_black(le) ? v3(m.base[0], m.base[1], m.base[2]) : le;
@@ -206,7 +210,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
   const BsdfParams<float> bs = params_of(ms);
   // per-vertex BSDF terms of the one-pass evaluations (one parameter set)
   BsdfPre pre{};
-  if (!rp.separate_sampling) pre = bsdf_pre(bp, wo);
+  if (!sep) pre = bsdf_pre(bp, wo);
   BsdfParams<Dual<kNumParams>> bd;
   if (ADJ) {
     adj->mat = sp.mat;
@@ -232,10 +236,10 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
     float pl = 0.f, s_tmax = 0.f;
     bool ok = false;
     if (pick_sun) {
-      wl = sample_sun(s.sky, u[2], u[3]);
+      wl = sample_sun(*s.sky, u[2], u[3]);
       if (dot(wl, sp.ng) > 0.f) {
-        lrad = sun_radiance(s.sky, wl);
-        pl = (sun_w / total) * s.sky.sun_pdf;
+        lrad = sun_radiance(*s.sky, wl);
+        pl = (sun_w / total) * s.sky->sun_pdf;
         s_tmax = 1e30f;
         ok = !is_black(lrad);
       }
@@ -265,8 +269,8 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
       const V3 wi = to_local(fr, wl);
       if (wi.z > 0.f) {
         float pb;
-        const T3<float> f = rp.separate_sampling ? bsdf_f(bp, wo, wi) : bsdf_f_pdf(bp, pre, wo, wi, pb);
-        if (rp.separate_sampling) pb = bsdf_pdf(bs, wo, wi);
+        const T3<float> f = sep ? bsdf_f(bp, wo, wi) : bsdf_f_pdf(bp, pre, wo, wi, pb);
+        if (sep) pb = bsdf_pdf(bs, wo, wi);
         // balance heuristic against BSDF sampling, whose sample would be traced by the
         // extension ray; the last vertex traces none, so there the light sample stands
         // alone (a partial weight would drop the BSDF share of the last bounce's light)
@@ -296,7 +300,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
     // with one parameter set, the sample's pdf and f come from one fused evaluation
     BsdfSample smp;
     T3<float> f1 = t3(0.f, 0.f, 0.f);
-    if (rp.separate_sampling) {
+    if (sep) {
       smp = bsdf_sample(bs, wo, u[4], u[5], u[6]);
     } else {
       smp.pdf = 0.f;
@@ -310,7 +314,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
     if (smp.pdf > 0.f) {
       const V3 wiw = normalize(to_world(fr, smp.wi));
       if (dot(wiw, sp.ng) > 0.f) {
-        const T3<float> f = rp.separate_sampling ? bsdf_f(bp, wo, smp.wi) : f1;
+        const T3<float> f = sep ? bsdf_f(bp, wo, smp.wi) : f1;
         const float k = smp.wi.z / smp.pdf;
         V3 Tn = v3(ps.T.x * f.x * k, ps.T.y * f.y * k, ps.T.z * f.z * k);
         bool alive = !is_black(Tn);
@@ -322,7 +326,7 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
           // carries earlier vertices' evaluation parameters), which the adjoint holds
           // fixed, so the continuation's derivative stays (df/dtheta) / f
           float q;
-          if (rp.separate_sampling) {
+          if (sep) {
             const T3<float> fs = bsdf_f(bs, wo, smp.wi);
             q = fmin(1.f, fmax(fmax(fs.x, fs.y), fs.z) * k);
           } else {
@@ -356,9 +360,10 @@ TRT_HD inline void shade_impl(const SceneView& s, const Material* sampling_mats,
 // One vertex: ps = incoming path state, h = closest hit of ray (ps.o, ps.d).
 // sampling_mats may differ from s.materials only in differentiable renders
 // (sampling decisions use it, BSDF values use s.materials).
+template <int SEP = -1>
 TRT_HD inline void shade(const SceneView& s, const Material* sampling_mats, const RenderParams& rp,
                          const PathState& ps, const Hit& h, uint32_t depth, ShadeOut& out) {
-  shade_impl<false>(s, sampling_mats, rp, ps, h, depth, out, v3(0.f), nullptr);
+  shade_impl<false, SEP>(s, sampling_mats, rp, ps, h, depth, out, v3(0.f), nullptr);
 }
 
 // Same vertex replayed for the adjoint: delta = dLoss/d(path radiance).

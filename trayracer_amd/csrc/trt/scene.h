@@ -98,7 +98,9 @@ struct SceneView {
   float total_power;
   uint32_t n_tris;
   uint32_t n_materials;
-  SkyParams sky;                // radiance of rays that leave the scene (disabled: black)
+  const SkyParams* sky;          // radiance of rays that leave the scene (disabled: black); never null.
+                                // A pointer, not a copy: read only on misses and sun samples, its
+                                // 18 words would otherwise sit in a kernel's scalar registers
 };
 
 struct Hit {
