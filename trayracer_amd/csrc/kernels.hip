@@ -1244,8 +1244,9 @@ struct ShadeList {
 // is off (path tracing), so only the slots that hit are listed (a miss at depth 0 still
 // writes its pixel's zero albedo and normal), and the wave shades 64 listed slots at a
 // time: in the 1M soup 18% of the rays miss (35% at the second bounce), lanes that the
-// per-slot loop left idle through the whole shading step. Held to 168 VGPRs, 3 waves per
-// SIMD (the allocator otherwise takes 172, 2 waves).
+// per-slot loop left idle through the whole shading step. Held to 3 waves per SIMD (150
+// VGPRs with one material set; the allocator could otherwise take more than 170, 2 waves;
+// 4 waves spill 20 bytes per lane and measured 1% slower).
 // SEP: the sampling materials are a set of their own (1) or the evaluation materials (0),
 // fixed per launch from RenderParams::separate_sampling.
 template <bool DEBUG, int SEP>
