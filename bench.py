@@ -248,6 +248,25 @@ def gather_ceilings():
     return out
 
 
+def load_instruction_costs():
+    """Time per wave-level 16-byte load instruction per CU (trt_measure_gather_lanes) for
+    48-byte gathers from an L1-resident buffer: every lane its own random record with 64,
+    32 and 8 active lanes, and all active lanes at one record with 64 and 8. Equal times
+    with 64 and 8 lanes at one record mean that an idle lane costs the load path as much
+    as an active one."""
+    import ctypes
+
+    from trayracer_amd import _lib
+    lib = _lib.load()
+    out = {}
+    for name, active, same in (("random_64_lanes", 64, 0), ("random_32_lanes", 32, 0), ("random_8_lanes", 8, 0),
+                               ("one_record_64_lanes", 64, 1), ("one_record_8_lanes", 8, 1)):
+        v = ctypes.c_double()
+        st = lib.trt_measure_gather_lanes(0, 16 << 10, 256, active, same, ctypes.byref(v))
+        out[name] = round(v.value, 2) if st == 0 else None
+    return out
+
+
 def extend_roofline(tc, info, ext_rays, ext_ms, launches, workload):
     """Two ceilings for the closest-hit traversal (Extend), per launch, from HIP-event
     launch times:
@@ -296,7 +315,8 @@ def extend_roofline(tc, info, ext_rays, ext_ms, launches, workload):
                           "tris_per_ray": round(tc["closest_tris_per_ray"], 2),
                           "loads_16B_per_ray": round(per_ray / 16.0, 1),
                           "peak_source": "trt_measure_gather: independent random 48-byte gathers, 16 KiB buffer",
-                          "gather_ceilings_gbs": ceil},
+                          "gather_ceilings_gbs": ceil,
+                          "load_instr_ns_per_cu": load_instruction_costs()},
         "stack_spills_per_ray": round(tc["closest_spills"] / max(1, tc["closest_rays"]), 4),
     }
 

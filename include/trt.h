@@ -377,6 +377,13 @@ trt_status trt_measure_traversal(trt_context* ctx, const trt_render_opts* opts, 
  * three timed launches). Over a buffer that fits the L2 this is the highest rate such
  * gathers reach on the device. No context: it runs on `device` with its own buffers. */
 trt_status trt_measure_gather(int device, uint64_t buffer_bytes, uint32_t iters, double* gbs);
+/* Measurement hook (no counterpart in the reference): the same loads timed per
+ * wave-level instruction, with only the first `active_lanes` (1..64) lanes of each wave
+ * reading, each lane its own pseudo-random record (same_record = 0) or every active
+ * lane of a wave the same one (1); *ns_per_load = nanoseconds per wave-level 16-byte load
+ * instruction per CU (best of three timed launches). */
+trt_status trt_measure_gather_lanes(int device, uint64_t buffer_bytes, uint32_t iters, uint32_t active_lanes,
+                                    uint32_t same_record, double* ns_per_load);
 
 /* Differentiable rendering. No counterpart in the reference (its PathTracingKernel,
  * Tracer/CUDA/GPU/Shade/PathTracing.h:8-175, has no adjoint): the adjoint of one frame

@@ -40,6 +40,12 @@ def test_null_and_invalid_arguments_return_status():
     assert lib.trt_build(None, None) == 1
     assert lib.trt_trace_closest(None, None, 0, None, None) == 1
     assert lib.trt_last_error(None) == b"null context"
+    v = ctypes.c_double()
+    assert lib.trt_measure_gather(0, 1 << 14, 1, None) == 1
+    assert lib.trt_measure_gather_lanes(0, 1 << 14, 1, 0, 0, ctypes.byref(v)) == 1   # no active lane
+    assert lib.trt_measure_gather_lanes(0, 1 << 14, 1, 65, 0, ctypes.byref(v)) == 1  # more lanes than a wave
+    assert lib.trt_measure_gather_lanes(0, 1 << 14, 1, 64, 2, ctypes.byref(v)) == 1  # unknown pattern
+    assert lib.trt_measure_gather_lanes(0, 16, 1, 64, 0, ctypes.byref(v)) == 1       # not one record
     assert lib.trt_version().startswith(b"trt")
     # no device in this container: creation reports a device error instead of aborting
     ctx = ctypes.c_void_p()

@@ -179,6 +179,8 @@ PROTOTYPES = [
                      ctypes.POINTER(TrtPickResult)]),
     ("trt_measure_traversal", S, [P, ctypes.POINTER(TrtRenderOpts), ctypes.POINTER(TrtTraceCounts)]),
     ("trt_measure_gather", S, [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    ("trt_measure_gather_lanes", S, [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.POINTER(ctypes.c_double)]),
     ("trt_set_material_params", S, [P, c_float_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("trt_backward", S, [P, ctypes.POINTER(TrtRenderOpts), c_float_p, c_float_p]),
     ("trt_denoiser_init", S, [P, ctypes.c_uint32]),
