@@ -1086,11 +1086,43 @@ __device__ __forceinline__ void store_primary(const RayQueue& q, uint32_t s, uin
   q.meta[s] = (p << 2) | ps.flags;
 }
 
+// Path whose primary ray RayGen thread t writes to slot t's place in the queue: the
+// frame's paths sample by sample, each sample's pixels in 8x8 tiles (row-major tiles over
+// the largest region with sides a multiple of 8, then the remaining bottom rows and right
+// columns row by row; a bijection on [0, W*H)), so that the 64 rays of a fetched batch
+// are a square of neighbouring pixels rather than 64 pixels of a row: the lanes of a wave
+// share more nodes on every load. Results are per path, whatever the queue order.
+__device__ __forceinline__ uint32_t tiled_path(uint32_t t, uint32_t W, uint32_t H) {
+  const uint32_t npix = W * H;
+  const uint32_t smp = t / npix, j = t - smp * npix;
+  const uint32_t W8 = W & ~7u, H8 = H & ~7u, core = W8 * H8;
+  uint32_t x, y;
+  if (j < core) {
+    const uint32_t tile = j >> 6, k = j & 63u, tiles_x = W8 >> 3;
+    x = (tile % tiles_x) * 8u + (k & 7u);
+    y = (tile / tiles_x) * 8u + (k >> 3);
+  } else {
+    uint32_t r = j - core;
+    const uint32_t bottom = (H - H8) * W;
+    if (r < bottom) {
+      y = H8 + r / W;
+      x = r % W;
+    } else {  // reached only when W8 < W
+      r -= bottom;
+      const uint32_t wr = W - W8;
+      y = r / wr;
+      x = W8 + r % wr;
+    }
+  }
+  return smp * npix + y * W + x;
+}
+
 __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams rp, uint32_t n_paths, uint32_t per, uint32_t cap,
                                                     RayQueue q, FrameBuffers fb, uint32_t* ctr) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   primary_counts(n_paths, per, ctr + seg_ctr(kSegExt, 0));
-  if (p < n_paths) {
+  if (t < n_paths) {
+    const uint32_t p = tiled_path(t, rp.width, rp.height);
     PathState ps;
     const bool alive = raygen(rp, p, ps);
     fb.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1098,7 +1130,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams rp, uint32_t n_p
       fb.path_aov_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
       fb.path_aov_n[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    store_primary(q, primary_slot(p, per, cap), p, ps, alive, rp.eps);
+    store_primary(q, primary_slot(t, per, cap), p, ps, alive, rp.eps);
   }
 }
 
