@@ -7,15 +7,18 @@
 // (CUDA/GPU/CudaUtilityKernels.h:6-44).
 //
 // Design:
-//  * Extend / Shadow: persistent waves fetch 64 rays per atomic from a queue of
-//    structure-of-arrays ray records; each lane traverses the BVH2 with a per-lane
-//    stack in LDS (lane-interleaved, bank-conflict free) sized to the tree depth.
+//  * Extend / Shadow: persistent waves refill idle lanes from a queue of
+//    structure-of-arrays ray records (one atomic per refill); each lane walks the BVH2
+//    (speculative while-while loop, leaf triangles dealt out over the wave) with a
+//    two-tier stack: a 16-entry LDS ring per lane (lane-interleaved) over a global spill
+//    area.
 //  * Shade: the shared integrator step (trt/integrator.h) per ray; extension and
 //    shadow rays are appended with one atomic per wave (ballot + mbcnt prefix) rather
 //    than one atomic per ray as in the reference (PathTracing.h:135-141, 168-173).
 //  * Queues are segmented (device.h: kSegs segments, each with its own counters on
 //    separate cache lines) so that fetch and append atomics spread over 32 lines.
-//  * RayGen writes path p to a fixed slot (no atomics): the primary queue is dense.
+//  * RayGen writes thread t's ray to a fixed slot (no atomics; the primary queue is
+//    dense), the paths of each sample in 8x8 pixel tiles.
 //  * No host synchronisation between bounces: queue sizes stay in device counters.
 //  * Radiance is accumulated per path (one writer per path per kernel) and reduced
 //    per pixel in a fixed sample order in Finalize, so images are deterministic.
