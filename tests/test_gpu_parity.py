@@ -259,10 +259,12 @@ def test_path_radiance_cornell_rr_deep(rend):
     _compare_paths(rend.path_radiance(), ref["path_rgb"], "cornell rr")
 
 
-@pytest.mark.parametrize("width,height,spp", [(1, 1, 1), (3, 5, 2), (67, 1, 3), (1, 130, 1)])
+@pytest.mark.parametrize("width,height,spp", [(1, 1, 1), (3, 5, 2), (67, 1, 3), (1, 130, 1), (67, 13, 2), (13, 67, 1)])
 def test_path_radiance_ragged_frames(rend, width, height, spp):
     """Frames whose path counts are far from a whole wave or queue segment (1 path up to
-    201), including single-row and single-column films."""
+    1742), including single-row and single-column films, and films whose sides are not
+    multiples of RayGen's 8x8 tiles (tiles, then the leftover bottom rows and right
+    columns: every path traced once, each with its own pixel's ray)."""
     sc = scenes.cornell(width=width, height=height, spp=spp, max_depth=4, rr_depth=3, seed=13)
     rend.UpdateScene(sc)
     rend.RenderFrame()
