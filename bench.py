@@ -324,18 +324,22 @@ def extend_roofline(tc, info, ext_rays, ext_ms, launches, workload):
 def shade_roofline(scene, stats, steps, info):
     """Shade's rooflines per frame. HBM: the bytes a path's vertex must stream at least,
     its queue record in (52 B: origin, direction, throughput, path), its hit (16 B, + 4 B
-    instance), the radiance read-modify-write (32 B), and the records it writes, an
-    extension (52 B) for each next-bounce ray and a shadow ray (48 B) for each shadow ray
-    emitted, over the frame's Shade time against 8 TB/s. The surface data it gathers
-    (triangle 36 B, + 36 B normals, material id 4 B, material 80 B, light record and alias
-    entry 88 B per shaded hit) is mostly served by the caches and is reported beside."""
+    instance), and the records it writes, an extension (52 B) for each next-bounce ray and
+    a shadow ray (48 B) for each shadow ray emitted, over the frame's Shade time against
+    8 TB/s. Shade writes no radiance (the shadow stream adds it); its emission-list
+    entries (16 B per vertex that reaches an emitter or the sky) are not counted, so the
+    figure is a lower bound. Shade's launches run beside the previous bounce's shadow
+    launch, whose tail holds CU slots, and their intervals include that time. The surface
+    data it gathers (triangle 36 B, + 36 B normals, material id 4 B, material 80 B, light
+    record and alias entry 88 B per shaded hit) is mostly served by the caches and is
+    reported beside."""
     two = bool(info["two_level"])
     nrm = scene.get("normals") is not None
     ext = sum(s["extend_rays"] for s in stats) / steps
     prim = sum(s["primary_rays"] for s in stats) / steps
     shadow = sum(s["shadow_rays"] for s in stats) / steps
     ms = sum(s["shade_ms"] for s in stats) / steps
-    stream = ext * (52 + 16 + (4 if two else 0) + 32) + (ext - prim) * 52 + shadow * 48
+    stream = ext * (52 + 16 + (4 if two else 0)) + (ext - prim) * 52 + shadow * 48
     gathered = 36 + (36 if nrm else 0) + 4 + 80 + 88
     achieved = stream / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -406,7 +410,9 @@ def main():
         "trace_mrays_s": round((ext_rays + sh_rays) / (trace_ms * 1e-3) / 1e6, 1),
         "extend_mrays_s": round(ext_rays / (ext_ms * 1e-3) / 1e6, 1),
         "shadow_mrays_s": round(sh_rays / (sh_ms * 1e-3) / 1e6, 1) if sh_ms > 0 else None,
-        "rates_note": "trace_mrays_s: Extend + Shadow rays over the wall time of both phases; extend_mrays_s and "
+        "rates_note": "trace_mrays_s: Extend + Shadow rays over trace_wall (frame minus RayGen, Shade and "
+                      "Finalize; Shade runs beside the previous bounce's Shadow, so that overlap counts as Shade "
+                      "time and trace_mrays_s overstates the two phases' rate by as much); extend_mrays_s and "
                       "shadow_mrays_s: each phase's rays over its own launch intervals, and every Shadow launch "
                       "shares the GPU with the next bounce's Extend (second stream), so these understate the "
                       "kernels' own rates",
@@ -414,7 +420,7 @@ def main():
         "phase_ms_per_frame": {"extend": round(ext_ms / args.steps, 3), "shade": round(shade_ms / args.steps, 3),
                                "shadow": round(sh_ms / args.steps, 3), "trace_wall": round(trace_ms / args.steps, 3),
                                "note": "extend and shadow are per-launch intervals that overlap across bounces; "
-                                       "trace_wall is the wall time of both phases"},
+                                       "trace_wall is the frame less RayGen, Shade and Finalize"},
         "bvh": {"nodes": info["nodes"], "depth": info["depth"], "sah_cost": round(info["sah_cost"], 2),
                 "presplit": args.presplit, "references": info["references"], "build_s": round(build_s, 2),
                 "trt_build_s": round(info["build_ms"] / 1e3, 2)},

@@ -135,16 +135,18 @@ typedef struct trt_stats {
   uint64_t primary_rays;
   float raygen_ms;
   float extend_ms;        /* sum over bounces of Extend kernel time (from launch to end) */
-  float shade_ms;
-  float shadow_ms;        /* Shadow of bounce d overlaps Extend of d + 1: from its launch to end */
+  float shade_ms;         /* sum over bounces of Shade, which runs beside the previous bounce's
+                             Shadow launch */
+  float shadow_ms;        /* Shadow of bounce d (its emission list included) overlaps Extend and
+                             Shade of d + 1: from its launch to end */
   float finalize_ms;
   float frame_ms;         /* first to last event of the frame */
   uint32_t extend_launches;
   uint32_t shadow_launches;
   uint32_t sample_count;  /* accumulated samples per pixel after this frame */
   uint32_t overflow;      /* nonzero if a traversal stack overflowed (must stay 0) */
-  float trace_ms;         /* wall time of the Extend and Shadow phases together (frame minus
-                             RayGen, Shade and Finalize, which run alone) */
+  float trace_ms;         /* frame minus RayGen, Shade and Finalize: the wall time of the Extend
+                             and Shadow phases, less the part of Shadow that runs beside Shade */
 } trt_stats;
 
 typedef struct trt_bvh_info {

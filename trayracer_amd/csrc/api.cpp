@@ -112,7 +112,9 @@ struct trt_context {
   // frame buffers
   DevBuf q_o[2], q_d[2], q_t[2], q_m[2];
   DevBuf h_t, h_i;
-  DevBuf s_o, s_d, s_c;
+  // shadow queues and emission lists, two of each: bounce d uses set d % 2 while the
+  // previous bounce's shadow launch still reads the other
+  DevBuf s_o[2], s_d[2], s_c[2], s_e[2];
   DevBuf radiance;
   DevBuf accum, color, albedo, normal;
   DevBuf aov_sum_a, aov_sum_n;    // per pixel AOV sums
@@ -308,9 +310,12 @@ trt_status ensure_frame(trt_context* ctx, uint32_t W, uint32_t H, uint32_t spp) 
     }
     HIPC(ctx->h_t.ensure(slots * 16), "alloc hits");
     HIPC(ctx->h_i.ensure(slots * 4), "alloc hits");
-    HIPC(ctx->s_o.ensure(slots * 16), "alloc shadow queue");
-    HIPC(ctx->s_d.ensure(slots * 16), "alloc shadow queue");
-    HIPC(ctx->s_c.ensure(slots * 16), "alloc shadow queue");
+    for (int i = 0; i < 2; ++i) {
+      HIPC(ctx->s_o[i].ensure(slots * 16), "alloc shadow queue");
+      HIPC(ctx->s_d[i].ensure(slots * 16), "alloc shadow queue");
+      HIPC(ctx->s_c[i].ensure(slots * 16), "alloc shadow queue");
+      HIPC(ctx->s_e[i].ensure(slots * 16), "alloc emission list");
+    }
     ctx->queue_slots = slots;
   }
   if (n > ctx->path_capacity) {
@@ -405,7 +410,10 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
     q[i].meta = ctx->q_m[i].as<uint32_t>();
   }
   HitBuf hb{ctx->h_t.as<float4>(), ctx->h_i.as<uint32_t>()};
-  ShadowQueue sq{ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>()};
+  ShadowQueue sq[2];
+  for (int i = 0; i < 2; ++i)
+    sq[i] = ShadowQueue{ctx->s_o[i].as<float4>(), ctx->s_d[i].as<float4>(), ctx->s_c[i].as<float4>(),
+                        ctx->s_e[i].as<float4>()};
   hipStream_t stm = ctx->stream;
   hipEvent_t* ev = ctx->ev;
   // debug modes: one bounce (two for ambient occlusion)
@@ -447,17 +455,22 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
       launch_touch_count(acc.touch_tris, ctx->touch_tri_words, ctr64 + 7, stm);
     }
     HIPC(hipEventRecord(ev[4 + 4 * d + 0], stm), "event");
-    // Shade adds to the radiance that the previous bounce's Shadow adds to, and reuses
-    // its shadow queue: wait for it
-    if (d > 0) HIPC(hipStreamWaitEvent(stm, ev2[2 * (d - 1) + 1], 0), "wait");
+    // Shade writes shadow queue and emission list d % 2, which the shadow launch of
+    // bounce d - 2 read: wait for it. It writes no path radiance, so it runs beside the
+    // previous bounce's shadow launch.
+    if (d > 1) HIPC(hipStreamWaitEvent(stm, ev2[2 * (d - 2) + 1], 0), "wait");
     HIPC(hipEventRecord(ev[4 + 4 * d + 3], stm), "event");
-    launch_shade(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq, fb,
+    launch_shade(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq[cur], fb,
                  SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkShade, di), cap},
-                 ctr + seg_ctr(kSegExt, di + 1), ctr + seg_ctr(kSegShadow, di), ctr, d, ctx->grid_shade, stm);
+                 ctr + seg_ctr(kSegExt, di + 1), ctr + seg_ctr(kSegShadow, di), ctr + seg_ctr(kSegEmit, di), ctr, d,
+                 ctx->grid_shade, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 1], stm), "event");
     HIPC(hipStreamWaitEvent(stm2, ev[4 + 4 * d + 1], 0), "wait");
     HIPC(hipEventRecord(ev2[2 * d], stm2), "event");
-    launch_shadow(acc_sh, sq, fb.radiance,
+    // all path radiance is added on this stream, in vertex order: the emission Shade
+    // found at bounce d after the shadow rays of bounce d - 1, before those of bounce d
+    launch_emit(sq[cur].emit, ctr + seg_ctr(kSegEmit, di), cap, fb.radiance, stm2);
+    launch_shadow(acc_sh, sq[cur], fb.radiance,
                   SegIn{ctr + seg_ctr(kSegShadow, di), ctr + seg_ctr(kSegWorkShadow, di), cap}, ctr, ctx->grid_trace,
                   stats, stm2);
     if (stats) {
@@ -1327,7 +1340,7 @@ trt_status trt_backward(trt_context* ctx, const trt_render_opts* o, const float*
     q[i].meta = ctx->q_m[i].as<uint32_t>();
   }
   HitBuf hb{ctx->h_t.as<float4>(), ctx->h_i.as<uint32_t>()};
-  ShadowQueue sq{ctx->s_o.as<float4>(), ctx->s_d.as<float4>(), ctx->s_c.as<float4>()};
+  ShadowQueue sq{ctx->s_o[0].as<float4>(), ctx->s_d[0].as<float4>(), ctx->s_c[0].as<float4>(), nullptr};
   hipStream_t stm = ctx->stream;
   const uint32_t cap = ctx->seg_cap;
   launch_zero_counters(ctr, stm);

@@ -26,11 +26,13 @@ struct HitBuf {
   uint32_t* inst;   // instance index (two-level scenes)
 };
 
-// Shadow-ray queue.
+// Shadow-ray queue, and beside it the emission list of the same bounce: radiance that
+// Shade found (emitter hits, sky) waiting to be added to its path.
 struct ShadowQueue {
   float4* o_tmin;
   float4* d_tmax;
   float4* contrib;  // rgb, path (bits)
+  float4* emit;     // emission list: rgb, path (bits); null where no Shade writes one
 };
 
 // Acceleration structure views.
@@ -76,8 +78,8 @@ constexpr int kCtrStats = 164;  // 10 x uint64: closest nodes, closest tris, any
                                 // nodes and triangle records read (summed over launches) by
                                 // closest and any traversals
 constexpr int kCtrSegBase = 256;
-enum SegKind : int { kSegExt = 0, kSegShadow = 1, kSegWorkExt = 2, kSegWorkShade = 3, kSegWorkShadow = 4 };
-constexpr int kNumSegKinds = 5;
+enum SegKind : int { kSegExt = 0, kSegShadow = 1, kSegWorkExt = 2, kSegWorkShade = 3, kSegWorkShadow = 4, kSegEmit = 5 };
+constexpr int kNumSegKinds = 6;
 constexpr int kSegDepths = kMaxDepth + 1;  // the last shade may address depth kMaxDepth
 constexpr int kNumCounters = kCtrSegBase + kNumSegKinds * kSegDepths * int(kSegs * kLine);
 constexpr int seg_ctr(int kind, int depth) { return kCtrSegBase + (kind * kSegDepths + depth) * int(kSegs * kLine); }
@@ -140,9 +142,15 @@ void launch_raygen(const RenderParams& rp, uint32_t n_paths, uint32_t cap, RayQu
                    uint32_t* ctr, hipStream_t stream);
 void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t* ctr, int grid, bool stats,
                    hipStream_t stream);
+// Shade writes no path radiance: what a vertex emits goes to the emission list sq.emit
+// (segmented, counters emit_cnt), which launch_emit adds to the paths.
 void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
                   HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, SegIn in, uint32_t* next_cnt,
-                  uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream);
+                  uint32_t* shadow_cnt, uint32_t* emit_cnt, uint32_t* ctr, uint32_t depth, int grid,
+                  hipStream_t stream);
+// Adds an emission list (kSegs segments of `cap` slots, counters emit_cnt) to the path
+// radiance; every path is listed at most once, so entries need no atomics.
+void launch_emit(const float4* emit, const uint32_t* emit_cnt, uint32_t cap, float4* radiance, hipStream_t stream);
 void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,
                    bool stats, hipStream_t stream);
 void launch_sum_counts(uint32_t* ctr, uint32_t cap, hipStream_t stream);

@@ -21,7 +21,10 @@
 //    dense), the paths of each sample in 8x8 pixel tiles.
 //  * No host synchronisation between bounces: queue sizes stay in device counters.
 //  * Radiance is accumulated per path (one writer per path per kernel) and reduced
-//    per pixel in a fixed sample order in Finalize, so images are deterministic.
+//    per pixel in a fixed sample order in Finalize, so images are deterministic. Shade
+//    lists emission rather than adding it; the emission and shadow launches of the
+//    shadow stream add every term in vertex order, and Shade runs beside the previous
+//    bounce's shadow launch.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -1302,8 +1305,9 @@ template <bool DEBUG, int SEP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(SceneView sv, const Material* sampling_mats, RenderParams rp,
                                                    RayQueue q, HitBuf h, RayQueue qn, ShadowQueue sq,
                                                    FrameBuffers fb, SegIn in, uint32_t* next_cnt,
-                                                   uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth,
-                                                   uint32_t two_level, const SkyParams* __restrict__ sky) {
+                                                   uint32_t* shadow_cnt, uint32_t* emit_cnt, uint32_t* ctr,
+                                                   uint32_t depth, uint32_t two_level,
+                                                   const SkyParams* __restrict__ sky) {
   // sv.sky again, as a restrict kernel argument: nothing the kernel writes can alias it, so
   // its fields are read with scalar loads where they are used
   sv.sky = sky;
@@ -1350,6 +1354,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     out.has_ext = 0;
     out.has_shadow = 0;
     uint32_t path = 0;
+    bool emits = false;
     if (lane < take) {
       const uint32_t slot = L.slot[lane];
       const float4 a = q.o_tmin[slot];
@@ -1374,17 +1379,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       hit.inst = hinst;
       if (DEBUG) shade_debug(sv, rp, ps, hit, depth, out);
       else shade<SEP>(sv, sampling_mats, rp, ps, hit, depth, out);
-      if (!is_black(out.L)) {  // rare (emitter hits): this lane is the path's only writer here
-        float4 Lr = fb.radiance[path];
-        Lr.x += out.L.x;
-        Lr.y += out.L.y;
-        Lr.z += out.L.z;
-        fb.radiance[path] = Lr;
-      }
+      emits = !is_black(out.L);
       if (depth == 0) {
         fb.path_aov_a[path] = make_float4(out.albedo.x, out.albedo.y, out.albedo.z, out.vdepth);
         fb.path_aov_n[path] = make_float4(out.normal.x, out.normal.y, out.normal.z, 0.f);
       }
+    }
+    // emission (emitter hits, the sky) to the emission list: the path radiance belongs to
+    // the previous bounce's shadow launch, which may still run
+    const uint64_t mL = __ballot(emits);
+    if (mL != 0ull) {
+      const uint32_t sl = seg_append(emit_cnt, in.cap, home, mL, &ctr[kCtrOverflow]);
+      if (emits && sl != kNoHit) sq.emit[sl] = make_float4(out.L.x, out.L.y, out.L.z, __uint_as_float(path));
     }
     // move the unprocessed tail (at most 63 slots, all at or above `take`) to the front
     if (take == 64u && lane + 64u < pend) {
@@ -1412,6 +1418,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       sq.d_tmax[s] = make_float4(out.s_d.x, out.s_d.y, out.s_d.z, out.s_tmax);
       sq.contrib[s] = make_float4(out.s_contrib.x, out.s_contrib.y, out.s_contrib.z, __uint_as_float(path));
     }
+  }
+}
+
+// A bounce's emission list into the path radiance. It runs on the shadow stream after
+// the previous bounce's shadow rays and before this bounce's, so every path's radiance
+// is summed in vertex order: emission at vertex d, then the next-event estimate of d.
+// Block (x, y) takes every gridDim.x-th group of kBlock entries of segment y.
+__global__ __launch_bounds__(kBlock) void k_emit(const float4* __restrict__ emit, const uint32_t* __restrict__ cnt,
+                                                 uint32_t cap, float4* radiance) {
+  const uint32_t sgm = blockIdx.y;
+  const uint32_t c = cnt[sgm * kLine];
+  const uint32_t n = c < cap ? c : cap;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float4 e = emit[size_t(sgm) * cap + i];
+    const uint32_t path = __float_as_uint(e.w);
+    float4 L = radiance[path];
+    L.x += e.x;
+    L.y += e.y;
+    L.z += e.z;
+    radiance[path] = L;
   }
 }
 
@@ -1774,16 +1800,23 @@ void launch_extend(const DevAccel& acc, RayQueue q, HitBuf h, SegIn in, uint32_t
 
 void launch_shade(const SceneView& sv, const Material* sampling_mats, const RenderParams& rp, RayQueue q,
                   HitBuf h, RayQueue qn, ShadowQueue sq, FrameBuffers fb, SegIn in, uint32_t* next_cnt,
-                  uint32_t* shadow_cnt, uint32_t* ctr, uint32_t depth, int grid, hipStream_t stream) {
+                  uint32_t* shadow_cnt, uint32_t* emit_cnt, uint32_t* ctr, uint32_t depth, int grid,
+                  hipStream_t stream) {
   if (rp.mode != kModePathTracing)
     hipLaunchKernelGGL((k_shade<true, -1>), dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq,
-                       fb, in, next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
+                       fb, in, next_cnt, shadow_cnt, emit_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
   else if (rp.separate_sampling)
     hipLaunchKernelGGL((k_shade<false, 1>), dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq,
-                       fb, in, next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
+                       fb, in, next_cnt, shadow_cnt, emit_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
   else
     hipLaunchKernelGGL((k_shade<false, 0>), dim3(grid), dim3(kBlock), 0, stream, sv, sampling_mats, rp, q, h, qn, sq,
-                       fb, in, next_cnt, shadow_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
+                       fb, in, next_cnt, shadow_cnt, emit_cnt, ctr, depth, sv.instances ? 1u : 0u, sv.sky);
+}
+
+void launch_emit(const float4* emit, const uint32_t* emit_cnt, uint32_t cap, float4* radiance, hipStream_t stream) {
+  // 16 blocks per segment: most lists are short (emitter hits); sky scenes list a third
+  // of the paths
+  hipLaunchKernelGGL(k_emit, dim3(16, kSegs), dim3(kBlock), 0, stream, emit, emit_cnt, cap, radiance);
 }
 
 void launch_shadow(const DevAccel& acc, ShadowQueue sq, float4* radiance, SegIn in, uint32_t* ctr, int grid,

@@ -83,7 +83,7 @@ TRT_FI bool raygen(const RenderParams& rp, uint32_t path, PathState& ps) {
 }
 
 struct ShadeOut {
-  V3 L;  // emission to add to the path radiance now
+  V3 L;  // emission at this vertex: added to the path radiance before s_contrib
   uint32_t has_shadow;
   uint32_t has_ext;
   V3 s_d;
