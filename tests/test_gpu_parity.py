@@ -2,8 +2,9 @@
 
 Bar: closest-hit primitive/instance ids, t and barycentrics bit-exact (the tie rule is
 order independent, so traversal order and tree shape cannot change them); any-hit
-flags exact; per-path radiance within 1e-4 relative (bit-exact expected: the shared
-arithmetic avoids contraction and libm calls on both sides).
+flags exact; per-path radiance bit-exact (the shared arithmetic avoids contraction and
+libm calls on both sides, and both add a path's terms in vertex order: the emission of
+vertex d, then its next-event estimate), with the relative error reported first.
 """
 import numpy as np
 import pytest
@@ -212,6 +213,8 @@ def _compare_paths(pg, pr, label, tol=1e-4):
     rel = np.abs(pg - pr) / denom
     bad = np.nonzero((rel > tol).any(axis=1))[0]
     assert bad.size == 0, f"{label}: {bad.size}/{pg.shape[0]} paths differ > {tol}, first {bad[:5]} gpu {pg[bad[:3]]} ref {pr[bad[:3]]}"
+    diff = np.nonzero((pg.view(np.uint32) != pr.view(np.uint32)).any(axis=1))[0]
+    assert diff.size == 0, f"{label}: {diff.size}/{pg.shape[0]} paths not bit-exact (within {tol}), first {diff[:5]}"
 
 
 def test_path_radiance_cornell(rend):
@@ -221,8 +224,6 @@ def test_path_radiance_cornell(rend):
     pg = rend.path_radiance()
     ref = _oracle(sc).render()
     _compare_paths(pg, ref["path_rgb"], "cornell")
-    exact = np.array_equal(pg.view(np.uint32), ref["path_rgb"].view(np.uint32))
-    print("bit-exact per-path radiance:", exact)
     # image = mean of paths in fixed sample order (same float ops as the oracle)
     img = rend.color()[..., :3]
     np.testing.assert_allclose(img, ref["image_f32"], rtol=1e-5, atol=1e-7)
