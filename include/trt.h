@@ -286,6 +286,13 @@ trt_status trt_set_sky(trt_context* ctx, const trt_sky* sky);
  * Tracer/Renderer/Renderer.h:45-61; DeviceRenderer::RenderFrame, Tracer/Renderer/DeviceRenderer.cpp:97-133). */
 trt_status trt_set_camera(trt_context* ctx, const trt_camera* cam);
 trt_status trt_render(trt_context* ctx, const trt_render_opts* opts, trt_stats* stats);
+/* Per-phase timing of trt_render (no counterpart in the reference, which brackets its
+ * frame with one pair of CUDA events, Tracer/Renderer/DeviceRenderer.cpp:119-125): on by
+ * default. Off, a frame records only the events its two streams need to stay ordered and
+ * the two that bracket it, since each event between two kernels delays the second by a
+ * few microseconds; trt_stats then holds the counts and frame_ms, and the per-phase
+ * times (raygen_ms ... finalize_ms, trace_ms) are 0. */
+trt_status trt_set_phase_timing(trt_context* ctx, int enabled);
 trt_status trt_reset(trt_context* ctx);
 uint32_t trt_sample_count(const trt_context* ctx);
 /* Number of frames this context has rendered (trt_render and trt_measure_traversal each

@@ -37,6 +37,7 @@ def test_null_and_invalid_arguments_return_status():
     lib = _lib.load()
     assert lib.trt_create(0, None) == 1  # TRT_INVALID_ARG
     assert lib.trt_render(None, None, None) == 1
+    assert lib.trt_set_phase_timing(None, 0) == 1
     assert lib.trt_build(None, None) == 1
     assert lib.trt_trace_closest(None, None, 0, None, None) == 1
     assert lib.trt_last_error(None) == b"null context"

@@ -232,6 +232,27 @@ def test_path_radiance_cornell(rend):
     np.testing.assert_array_equal(rend.normal(), aov[..., 4:8])
 
 
+def test_phase_timing_off_changes_no_result(rend):
+    """A frame rendered without the per-phase timing events (only those that order the two
+    streams) has the same paths, counts and image as one with them; its phase times are 0."""
+    sc = scenes.cornell(width=48, height=40, spp=2, max_depth=5, rr_depth=3, seed=11)
+    rend.UpdateScene(sc)
+    on = rend.RenderFrame()
+    p_on = rend.path_radiance().copy()
+    rend.Reset()
+    rend.SetPhaseTiming(False)
+    try:
+        off = rend.RenderFrame()
+    finally:
+        rend.SetPhaseTiming(True)
+    assert np.array_equal(rend.path_radiance().view(np.uint32), p_on.view(np.uint32))
+    for k in ("paths", "extend_rays", "shadow_rays", "primary_rays", "extend_launches", "overflow"):
+        assert off[k] == on[k], k
+    assert off["frame_ms"] > 0 and on["shade_ms"] > 0
+    for k in ("raygen_ms", "extend_ms", "shade_ms", "shadow_ms", "finalize_ms", "trace_ms"):
+        assert off[k] == 0.0, k
+
+
 def test_aov_means_accumulate_over_frames(rend):
     """Albedo / normal / view depth are per-pixel means over every accumulated sample:
     two frames of 3 samples equal the oracle's 6-sample frame (up to the sums'

@@ -163,6 +163,7 @@ PROTOTYPES = [
     ("trt_set_camera", S, [P, ctypes.POINTER(TrtCamera)]),
     ("trt_set_sky", S, [P, ctypes.POINTER(TrtSky)]),
     ("trt_render", S, [P, ctypes.POINTER(TrtRenderOpts), ctypes.POINTER(TrtStats)]),
+    ("trt_set_phase_timing", S, [P, ctypes.c_int]),
     ("trt_reset", S, [P]),
     ("trt_sample_count", ctypes.c_uint32, [P]),
     ("trt_frame_id", ctypes.c_uint64, [P]),

@@ -141,6 +141,7 @@ struct trt_context {
   DevBuf d_mat_sampling;
   DevBuf d_sky;  // `sky` as the kernels read it (SceneView::sky)
   bool has_frame = false;      // the last frame's per-path radiance matches the current scene
+  bool phase_timing = true;    // record the per-phase timing events (trt_set_phase_timing)
   uint64_t frame_id = 0;       // frames rendered by this context
   trt_render_opts last_opts{};
   uint32_t last_sample_offset = 0;
@@ -441,10 +442,15 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
   hipStream_t stm2 = ctx->stream2;
   hipEvent_t* ev2 = ctx->ev2;
 
+  // Events: ev[4 + 4d + 1] (Shade done) and ev2[2d + 1] (Shadow done) order the two
+  // streams, ev[0] and ev[2] bracket the frame; the others only time the phases. Every
+  // event recorded between two kernels of a stream delays the second by a few
+  // microseconds, so those are left out when phase timing is off.
+  const bool timing = ctx->phase_timing;
   launch_zero_counters(ctr, stm);
   HIPC(hipEventRecord(ev[0], stm), "event");
   launch_raygen(rp, n, cap, q[0], fb, ctr, stm);
-  HIPC(hipEventRecord(ev[1], stm), "event");
+  if (timing) HIPC(hipEventRecord(ev[1], stm), "event");
   for (uint32_t d = 0; d < D; ++d) {
     const int cur = int(d & 1u), nxt = cur ^ 1;
     const int di = int(d);
@@ -454,19 +460,19 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
       launch_touch_count(acc.touch_nodes, ctx->touch_node_words, ctr64 + 6, stm);
       launch_touch_count(acc.touch_tris, ctx->touch_tri_words, ctr64 + 7, stm);
     }
-    HIPC(hipEventRecord(ev[4 + 4 * d + 0], stm), "event");
+    if (timing) HIPC(hipEventRecord(ev[4 + 4 * d + 0], stm), "event");
     // Shade writes shadow queue and emission list d % 2, which the shadow launch of
     // bounce d - 2 read: wait for it. It writes no path radiance, so it runs beside the
     // previous bounce's shadow launch.
     if (d > 1) HIPC(hipStreamWaitEvent(stm, ev2[2 * (d - 2) + 1], 0), "wait");
-    HIPC(hipEventRecord(ev[4 + 4 * d + 3], stm), "event");
+    if (timing) HIPC(hipEventRecord(ev[4 + 4 * d + 3], stm), "event");
     launch_shade(sv, sampling_mats(ctx), rp, q[cur], hb, q[nxt], sq[cur], fb,
                  SegIn{ctr + seg_ctr(kSegExt, di), ctr + seg_ctr(kSegWorkShade, di), cap},
                  ctr + seg_ctr(kSegExt, di + 1), ctr + seg_ctr(kSegShadow, di), ctr + seg_ctr(kSegEmit, di), ctr, d,
                  ctx->grid_shade, stm);
     HIPC(hipEventRecord(ev[4 + 4 * d + 1], stm), "event");
     HIPC(hipStreamWaitEvent(stm2, ev[4 + 4 * d + 1], 0), "wait");
-    HIPC(hipEventRecord(ev2[2 * d], stm2), "event");
+    if (timing) HIPC(hipEventRecord(ev2[2 * d], stm2), "event");
     // all path radiance is added on this stream, in vertex order: the emission Shade
     // found at bounce d after the shadow rays of bounce d - 1, before those of bounce d
     launch_emit(sq[cur].emit, ctr + seg_ctr(kSegEmit, di), cap, fb.radiance, stm2);
@@ -480,7 +486,7 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
     HIPC(hipEventRecord(ev2[2 * d + 1], stm2), "event");
   }
   if (D > 0) HIPC(hipStreamWaitEvent(stm, ev2[2 * (D - 1) + 1], 0), "wait");
-  HIPC(hipEventRecord(ev[3], stm), "event");
+  if (timing) HIPC(hipEventRecord(ev[3], stm), "event");
   if (accumulate)
     launch_finalize(rp, fb.radiance, fb, ctx->sample_count == 0 ? 1u : 0u, ctx->aov_count == 0 ? 1u : 0u,
                     ctx->aov_count + o.spp, stm);
@@ -509,25 +515,27 @@ trt_status run_frame(trt_context* ctx, const trt_render_opts& o, bool stats, boo
       st->shadow_rays += hc[kCtrShadow + d];
     }
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-    st->raygen_ms = ms;
-    // per-kernel intervals; Extend and Shadow overlap, so the phases add up to more than
-    // the frame
-    hipEvent_t prev = ev[1];
-    for (uint32_t d = 0; d < D; ++d) {
-      (void)hipEventElapsedTime(&ms, prev, ev[4 + 4 * d + 0]);
-      st->extend_ms += ms;
-      (void)hipEventElapsedTime(&ms, ev[4 + 4 * d + 3], ev[4 + 4 * d + 1]);
-      st->shade_ms += ms;
-      (void)hipEventElapsedTime(&ms, ev2[2 * d], ev2[2 * d + 1]);
-      st->shadow_ms += ms;
-      prev = ev[4 + 4 * d + 1];
-    }
-    (void)hipEventElapsedTime(&ms, D > 0 ? ev[3] : ev[1], ev[2]);
-    st->finalize_ms = ms;
     (void)hipEventElapsedTime(&ms, ev[0], ev[2]);
     st->frame_ms = ms;
-    st->trace_ms = st->frame_ms - st->raygen_ms - st->shade_ms - st->finalize_ms;
+    if (timing) {
+      (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+      st->raygen_ms = ms;
+      // per-kernel intervals; Extend and Shadow overlap, so the phases add up to more
+      // than the frame
+      hipEvent_t prev = ev[1];
+      for (uint32_t d = 0; d < D; ++d) {
+        (void)hipEventElapsedTime(&ms, prev, ev[4 + 4 * d + 0]);
+        st->extend_ms += ms;
+        (void)hipEventElapsedTime(&ms, ev[4 + 4 * d + 3], ev[4 + 4 * d + 1]);
+        st->shade_ms += ms;
+        (void)hipEventElapsedTime(&ms, ev2[2 * d], ev2[2 * d + 1]);
+        st->shadow_ms += ms;
+        prev = ev[4 + 4 * d + 1];
+      }
+      (void)hipEventElapsedTime(&ms, D > 0 ? ev[3] : ev[1], ev[2]);
+      st->finalize_ms = ms;
+      st->trace_ms = st->frame_ms - st->raygen_ms - st->shade_ms - st->finalize_ms;
+    }
     st->extend_launches = D;
     st->shadow_launches = D;
     st->sample_count = ctx->sample_count;
@@ -546,6 +554,12 @@ trt_status trt_internal_fail(trt_context* ctx, trt_status s, const std::string& 
 extern "C" {
 
 const char* trt_version(void) { return "trt 0.1 (gfx950 wavefront path tracer)"; }
+
+trt_status trt_set_phase_timing(trt_context* ctx, int enabled) {
+  if (!ctx) return TRT_INVALID_ARG;
+  ctx->phase_timing = enabled != 0;
+  return TRT_OK;
+}
 
 static trt_status init_context(trt_context* ctx) {
   HIPC(hipSetDevice(ctx->device), "hipSetDevice");

@@ -202,6 +202,11 @@ class Renderer:
         self.last_stats = st.as_dict()
         return self.last_stats
 
+    def SetPhaseTiming(self, enabled):
+        """Per-phase times in RenderFrame's statistics (on by default). Off, a frame
+        records only the events that order its two streams; counts and frame_ms remain."""
+        self._check(self.lib.trt_set_phase_timing(self.ctx, 1 if enabled else 0))
+
     def Reset(self):
         self._check(self.lib.trt_reset(self.ctx))
 
