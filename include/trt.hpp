@@ -131,6 +131,8 @@ class Renderer {
     return stats_;
   }
   void Reset() { check(trt_reset(ctx_)); }
+  // Per-phase times in Statistics() (on by default; off, only counts and frame_ms).
+  void SetPhaseTiming(bool on) { check(trt_set_phase_timing(ctx_, on ? 1 : 0)); }
   const trt_stats& Statistics() const { return stats_; }
   uint32_t SampleCount() const { return trt_sample_count(ctx_); }
 
